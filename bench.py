@@ -1,0 +1,139 @@
+"""Benchmark of the MI355X-native AnisoMPM step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
+                    [--cpu-steps 2] [--no-cpu-baseline]
+
+Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
+(binning, stress/damage, P2G, grid update, G2P, advection) on particle state
+already resident in HBM; ``value`` = particles x steps / wall seconds of the
+K timed steps (bracketed by torch.cuda.synchronize()).
+
+``roofline`` is for the dominant kernel, timed live with HIP events recorded
+by libpmpm on the launch stream (pmpm_timing): achieved = its algorithmic
+bytes per launch / its mean launch time, against MI355X's 8 TB/s HBM peak.
+Algorithmic bytes per launch (fp32, damage on):
+  stress+P2G  238 B per particle + 24 B per active grid node
+  G2P         152 B per particle + 16 B per active grid node
+  grid update  40 B per active grid node
+  binning      32 B per particle
+``cpu_baseline`` times the oracle (an op-for-op torch-CPU restatement of the
+reference step, bit-identical to it on the CPU) on this host, on a bounded
+sample of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle-steps/s at 1M fp32, batched vs plain; adjoint/forward time ratio"
+HBM_PEAK_GBS = 8000.0
+KERNELS = ("binning", "stress_p2g", "grid_update", "g2p")
+
+
+def algorithmic_bytes(kernel, n, n_act):
+    return {"binning": 32 * n, "stress_p2g": 238 * n + 24 * n_act, "grid_update": 40 * n_act,
+            "g2p": 152 * n + 16 * n_act}[kernel]
+
+
+def make_scene(name):
+    from peel_mpm_amd import scenes
+    return {"config0": scenes.config0, "config1": scenes.config1, "config2": scenes.config2}[name]()
+
+
+def time_gpu(spec, steps, warmup):
+    import torch
+    from peel_mpm_amd import AnisoMPM, _lib, scenes
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
+    L = _lib.lib()
+    for s in range(warmup):
+        sim.substep(s * spec["dt"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        sim.substep((warmup + s) * spec["dt"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # per-kernel live timing over a second pass of the same number of steps
+    L.pmpm_timing(1)
+    for s in range(steps):
+        sim.substep((warmup + steps + s) * spec["dt"])
+    torch.cuda.synchronize()
+    ms = (C.c_double * 4)()
+    cnt = (C.c_int64 * 4)()
+    _lib.check(L.pmpm_timing_read(ms, cnt, 4), "pmpm_timing_read")
+    L.pmpm_timing(0)
+    n_q, n_c = sim.last_step_stats()
+    finite = bool(torch.isfinite(sim.x).all())
+    per_kernel = {k: (ms[i] / max(cnt[i], 1)) for i, k in enumerate(KERNELS)}
+    return sim.n, wall, per_kernel, (n_q, n_c), finite
+
+
+def time_cpu(spec, steps):
+    import torch
+    from oracle import cpu_step as O
+    from peel_mpm_amd import scenes
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    torch.set_num_threads(threads)
+    sim = scenes.make_solver(O.OracleMPM, spec, device="cpu")
+    sim.substep(0.0)                       # warm-up
+    t0 = time.perf_counter()
+    for s in range(steps):
+        sim.substep((1 + s) * spec["dt"])
+    wall = time.perf_counter() - t0
+    return sim.n * steps / wall, torch.get_num_threads(), wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2"))
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the step has no multi-GPU path in this build (--gpus {args.gpus}); "
+              "scenes shard across GPUs as independent processes", file=sys.stderr)
+        sys.exit(2)
+
+    spec = make_scene(args.workload)
+    n, wall, per_kernel, (n_q, n_c), finite = time_gpu(spec, args.steps, args.warmup)
+    value = n * args.steps / wall
+    dominant = max(per_kernel, key=per_kernel.get)
+    n_act = max(n_q, n_c)
+    alg = algorithmic_bytes(dominant, n, n_act)
+    achieved = alg / (per_kernel[dominant] * 1e-3) / 1e9
+    step_bytes = 289 * n + 32 * n_q + 16 * n_c
+    out = {
+        "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{args.workload}: {n} particles, {spec['n_grid']}^3 grid, dt={spec['dt']}",
+                   "particles": n, "n_grid": spec["n_grid"], "dt": spec["dt"], "form": "plain single-scene"},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg},
+        "kernel_ms": {k: round(v, 5) for k, v in per_kernel.items()},
+        "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
+                          "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+        "active_nodes": {"quadratic": n_q, "cubic": n_c},
+        "finite": finite,
+    }
+    if not args.no_cpu_baseline and args.cpu_steps > 0:
+        cv, cores, cwall = time_cpu(spec, args.cpu_steps)
+        out["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port",
+                               "sample": f"oracle (torch-CPU restatement of the reference step) on the same "
+                                         f"{args.workload} scene: 1 warm-up + {args.cpu_steps} timed steps, "
+                                         f"{cwall:.1f} s"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,166 @@
+/*
+ * pmpm.h -- C ABI of the MI355X-native AnisoMPM substep (libpmpm.so).
+ *
+ * The reference has no FFI: its step is the Python method
+ * AnisoMPM.substep (anisompm_py/src/anisompm.py:593-803).  Each entry point
+ * below replaces one piece of that method (or of its helpers) and is what a
+ * Python ctypes binding calls; the binding itself is peel_mpm_amd/_lib.py.
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers borrowed from the caller
+ *     (PyTorch tensors); the library never allocates device memory.  The
+ *     caller also provides the workspace (pmpm_workspace_bytes) and calls
+ *     pmpm_workspace_init once before the first step on it.
+ *   - Particle arrays keep the reference's row-major per-particle layout:
+ *     vectors (N,3), matrices (N,3,3), scalars (N,), flags (N,) uint8.
+ *   - dtype is PMPM_F32 or PMPM_F64 and applies to every floating array.
+ *   - stream is a hipStream_t passed as void* (NULL = default stream); all
+ *     work is enqueued on it and the call returns without synchronising.
+ *   - Every entry point returns 0 on success and a negative PMPM_E* code on
+ *     failure; pmpm_last_error() then describes it.  Nothing throws.
+ *   - Not thread-safe per workspace; two steps on one workspace must be
+ *     ordered on one stream.
+ */
+#ifndef PMPM_H
+#define PMPM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PMPM_F32 = 0, PMPM_F64 = 1 };
+enum { PMPM_OK = 0, PMPM_EARG = -1, PMPM_EHIP = -2, PMPM_ESIZE = -3 };
+
+enum { PMPM_MAX_COLLIDERS = 8, PMPM_MAX_VEL_BC = 8 };
+enum { PMPM_COLLIDER_SPHERE = 1, PMPM_COLLIDER_HALFSPACE = 2 };
+enum { PMPM_STICKY = 0, PMPM_SLIP = 1 };
+
+/* One grid collider, evaluated at the step's time (anisompm.py:828-867).
+ * p0 = centre/origin at time t, p1 = at t + 1e-4 (the collider velocity is
+ * the forward difference (p1 - p0) / 1e-4, formed on the device in dtype). */
+typedef struct {
+    int32_t kind;        /* PMPM_COLLIDER_SPHERE or PMPM_COLLIDER_HALFSPACE */
+    int32_t mode;        /* PMPM_STICKY or PMPM_SLIP */
+    double friction;
+    double radius;       /* sphere only */
+    double p0[3];
+    double p1[3];
+    double normal[3];    /* half-space only: unit outward normal */
+} pmpm_collider;
+
+/* Step configuration (AnisoMPM.__init__ kwargs, anisompm.py:304-327, plus
+ * the host-side part of the damage gate, anisompm.py:647). */
+typedef struct {
+    int32_t dtype;
+    int32_t n_grid;
+    double grid_lim, dx, inv_dx, dt;
+    double gravity[3];
+    double grid_damp;
+    int32_t use_fclamp;          /* f_clamp is not None */
+    double fclamp_lo, fclamp_hi;
+    int32_t do_damage;           /* allow.any() and step % damage_every == 0 */
+    double dt_damage;            /* dt * damage_every */
+    int32_t n_colliders;
+    pmpm_collider colliders[PMPM_MAX_COLLIDERS];
+    int32_t n_vel_bc;            /* velocity-BC table used with bc_id (1-based) */
+    double vel_bc[PMPM_MAX_VEL_BC][3];
+    int32_t stable_sort;         /* 1: within-bucket order = ascending particle index */
+    int32_t export_active;       /* 1: write the dense active-node mask (slow colliders) */
+    int32_t exact_sigma_eig;     /* 1: Cauchy-stress eigenpairs by a second Jacobi, as the
+                                    reference (anisompm.py:653); 0: closed form from the
+                                    polar stretch, which shares its eigenvectors */
+} pmpm_step_params;
+
+/* Particle state and materials at the start of the step (AnisoMPM attributes,
+ * anisompm.py:374-463). n_interface is read only where interface_mask != 0.
+ * bc_id may be NULL; else a nonzero bc_id[p] = k replaces v[p] by vel_bc[k-1]
+ * for the transfer, as the reference's particle_bc hooks do (anisompm.py:601). */
+typedef struct {
+    int64_t n;
+    const void *x, *v, *F, *C, *d, *lap, *D_hist;
+    const void *vol0, *mass, *mu, *lam, *sigma_c, *eta, *zeta, *l0, *residual, *A0;
+    const uint8_t *allow, *interface_mask;
+    const void *n_interface;
+    const uint8_t *bc_id;
+} pmpm_particles_in;
+
+/* Outputs.  x, v, F, C, lap, tau_last, J_last are always written.  d, D_hist,
+ * phi, phi_shear are written only when do_damage (the reference then rebinds
+ * them); otherwise they may be NULL.  Outputs must not alias inputs. */
+typedef struct {
+    void *x, *v, *F, *C, *lap, *d, *D_hist, *phi, *phi_shear, *tau_last, *J_last;
+} pmpm_particles_out;
+
+/* ---- workspace ---------------------------------------------------------- */
+/* Bytes of workspace for up to max_n particles on an n_grid^3 grid. */
+size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype);
+/* Zero the parts of the workspace whose zero state the step maintains. */
+int pmpm_workspace_init(void *ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t dtype,
+                        void *stream);
+
+/* ---- the step ------------------------------------------------------------ */
+/* One substep: AnisoMPM.substep (anisompm.py:593-803) minus the Python
+ * particle_bc hooks (replaced by bc_id/vel_bc) and with device colliders. */
+int pmpm_substep(const pmpm_step_params *p, const pmpm_particles_in *in,
+                 const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
+/* The same step split at the collider call (anisompm.py:736-746) so a host
+ * may apply arbitrary collider callables to the grid velocity in between:
+ * begin = bin + stress/damage + P2G + grid update (+ device colliders);
+ * end   = G2P + cubic damage Laplacian + advection/F update/clamp. */
+int pmpm_substep_begin(const pmpm_step_params *p, const pmpm_particles_in *in,
+                       const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
+int pmpm_substep_end(const pmpm_step_params *p, const pmpm_particles_in *in,
+                     const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
+/* Byte offsets into ws of the grid velocity (n_grid^3 x 4: vx,vy,vz,gd) and
+ * of the dense active-node mask (n_grid^3 uint8, valid after begin with
+ * export_active=1), for a host that resolves colliders itself. */
+int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t dtype, size_t *gv_offset,
+                    size_t *active_offset);
+/* Active quadratic nodes (grid_m > 1e-12) and cubic nodes (sum w > 1e-12) of
+ * the last step on ws; synchronises the stream. */
+int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t dtype,
+                         int64_t *n_active_q, int64_t *n_active_c, void *stream);
+
+/* ---- pieces exposed for function-level parity --------------------------- */
+/* Binning: bucket key = 4x4x4-cell block of floor(x*inv_dx) (clamped to the
+ * grid), counting sort by key.  perm (N) lists particles bucket by bucket;
+ * with stable != 0 it equals a stable argsort of key.  key (N) and
+ * bucket_start ((n_grid/4 rounded up)^3 + 1) may be NULL.  The cells are
+ * the reference's base cells (anisompm.py:549, :574). */
+int pmpm_bin(const void *x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t stable,
+             int32_t *key, int32_t *perm, int32_t *bucket_start, void *ws, size_t ws_bytes,
+             void *stream);
+/* Stencil of each particle as the step computes it: quadratic base cell and
+ * 27 node indices, cubic base cell and 64 node indices (clamp-to-edge), and
+ * the weights (anisompm.py:544-590).  Any output may be NULL.
+ * base_q, base_c: (N,3) int32; idx27 (N,27), idx64 (N,64) int32;
+ * w (N,3,3), wc (N,4,3), wc2 (N,4,3) in dtype. */
+int pmpm_stencil(const void *x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx,
+                 int32_t *base_q, int32_t *idx27, int32_t *base_c, int32_t *idx64, void *w,
+                 void *wc, void *wc2, void *stream);
+/* Register-resident batched cyclic Jacobi on symmetric 3x3 (anisompm.py:48-95):
+ * A (N,3,3) -> evals (N,3) ascending, V (N,3,3) eigenvectors as columns. */
+int pmpm_sym_eig(const void *A, int32_t dtype, int64_t n, void *evals, void *V, void *stream);
+
+/* Live kernel timing with HIP events on the launch stream (for benchmarks):
+ * pmpm_timing(1) clears and starts recording around each kernel group
+ * (0 binning, 1 stress+P2G, 2 grid update, 3 G2P); pmpm_timing_read
+ * synchronises and returns each group's summed milliseconds and launch
+ * count (n <= 4 groups).  Adds two event records per group per step. */
+int pmpm_timing(int32_t enable);
+int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
+
+const char *pmpm_last_error(void);
+const char *pmpm_version(void);
+/* sizeof of pmpm_collider, pmpm_step_params, pmpm_particles_in,
+ * pmpm_particles_out (in that order) for binding-layout checks; returns the
+ * number of sizes written (at most n). */
+int pmpm_abi_sizes(size_t *sizes, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PMPM_H */

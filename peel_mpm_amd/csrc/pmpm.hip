@@ -1,0 +1,1172 @@
+// libpmpm: MI355X (gfx950) kernels of the AnisoMPM substep and their C ABI.
+//
+// One step (anisompm_py/src/anisompm.py:593-803) runs as:
+//   k_bin_count   key = 4x4x4 cell-block of each particle, per-bucket slots
+//                 (wave-aggregated atomics over runs of equal keys)
+//   k_scan_tiles / k_scan_apply
+//                 bucket offsets (exclusive scan), ordered list of non-empty
+//                 buckets, marks on the grid blocks their stencils reach
+//   k_bin_scatter perm[start[key]+slot] = particle   (+ k_bin_stabilize)
+//   k_mark_compact list of marked grid blocks (clears the marks)
+//   k_p2g         per non-empty bucket: stress/damage/degradation in
+//                 registers, then quadratic momentum and cubic damage
+//                 scatter into a 7x7x7-node LDS arena (LDS float atomics),
+//                 stored to the bucket's HBM arena slot with plain
+//                 coalesced stores -- no HBM atomics
+//   k_grid        per marked 4x4x4 node block: each node sums the <= 8
+//                 bucket arenas that cover it in a fixed order, then
+//                 velocity, gravity, damping, colliders, cubic normalisation
+//   k_g2p         per non-empty bucket: LDS-staged arena gather, velocity,
+//                 affine C, cubic Laplacian, advection, F update, clamp
+// Particle arrays stay in the caller's order; kernels visit them through perm.
+#include <hip/hip_runtime.h>
+
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/pmpm.h"
+#include "pmpm_device.h"
+
+using namespace pmpm;
+
+namespace {
+
+constexpr int BLK = 4;                 // cells per bucket edge (= nodes per grid-block edge)
+constexpr int AR = BLK + 3;            // arena edge: nodes [4b-1, 4b+5]
+constexpr int AR3 = AR * AR * AR;      // 343
+constexpr int WG = 256;                // threads per work-group
+constexpr int SCAN_TILE = 1024;        // buckets per scan tile (4 per thread)
+constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket kernels
+constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
+    snprintf(g_err, sizeof g_err, fmt, a, b);
+    return code;
+}
+#define HIPCHK(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) return fail(PMPM_EHIP, "%s (code %lld)", hipGetErrorString(e_), (long long)e_); \
+    } while (0)
+
+inline int nblk(int ng) { return (ng + BLK - 1) / BLK; }
+
+// Optional live timing: HIP events recorded on the launch stream around each
+// kernel group, summed on read.  Groups: 0 binning, 1 p2g, 2 grid, 3 g2p.
+constexpr int NTIMED = 4;
+struct Timing {
+    bool on = false;
+    std::vector<hipEvent_t> ev[NTIMED];   // start/stop pairs
+    std::vector<hipEvent_t> pool;
+    hipEvent_t get() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    void reset() {
+        for (auto& v : ev) { for (auto e : v) pool.push_back(e); v.clear(); }
+    }
+};
+Timing g_timing;
+struct TimedScope {
+    int group;
+    hipStream_t st;
+    hipEvent_t stop = nullptr;
+    TimedScope(int g, hipStream_t s) : group(g), st(s) {
+        if (!g_timing.on) return;
+        hipEvent_t start = g_timing.get();
+        stop = g_timing.get();
+        if (!start || !stop) { stop = nullptr; return; }
+        hipEventRecord(start, st);
+        g_timing.ev[group].push_back(start);
+    }
+    ~TimedScope() {
+        if (!stop) return;
+        hipEventRecord(stop, st);
+        g_timing.ev[group].push_back(stop);
+    }
+};
+inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
+
+// Workspace layout (byte offsets).  The step keeps count and mark at zero
+// between steps: whoever reads them clears them.  arena holds one 6 x 343
+// slot per non-empty bucket (m, mv, sum w d, sum w), in active-list order.
+struct Layout {
+    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, total;
+    long long NB, ntiles, G, max_slots;
+    Layout(int64_t n, int ng, int dtype) {
+        size_t ts = dtype == PMPM_F64 ? 8 : 4;
+        G = (long long)ng * ng * ng;
+        NB = (long long)nblk(ng) * nblk(ng) * nblk(ng);
+        ntiles = (NB + SCAN_TILE - 1) / SCAN_TILE;
+        max_slots = NB < (long long)(n > 0 ? n : 1) ? NB : (long long)(n > 0 ? n : 1);
+        size_t o = 0;
+        arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
+        gv = o;     o = al(o + (size_t)G * 4 * ts);
+        active = o; o = al(o + (size_t)G);
+        count = o;  o = al(o + (size_t)NB * 4);
+        start = o;  o = al(o + (size_t)(NB + 1) * 4);
+        tiles = o;  o = al(o + (size_t)ntiles * 2 * 4);
+        alist = o;  o = al(o + (size_t)NB * 4);
+        slotof = o; o = al(o + (size_t)NB * 4);
+        mark = o;   o = al(o + (size_t)NB * 4);
+        blist = o;  o = al(o + (size_t)NB * 4);
+        key = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);
+        slot = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
+        perm = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
+        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2..3] stats (u64 x2)
+        total = o;
+    }
+};
+
+// ------------------------------------------------------------------ binning
+template <typename T>
+__device__ __forceinline__ int bucket_of(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb) {
+    int b[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int c = floor_idx(scaled(x[3 * p + a], inv_dx), ng);
+        b[a] = clampi(c >> 2, 0, nb - 1);
+    }
+    return (b[0] * nb + b[1]) * nb + b[2];
+}
+
+// Slot within the bucket: one atomic per run of equal keys in the wavefront.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long long n, T inv_dx, int ng, int nb,
+                                                  int* __restrict__ count, int* __restrict__ key,
+                                                  int* __restrict__ slot) {
+    long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool live = p < n;
+    int k = live ? bucket_of(x, p, inv_dx, ng, nb) : -1;
+    int kprev = __shfl_up(k, 1);
+    bool head = live && (lane == 0 || kprev != k);
+    unsigned long long heads = __ballot(head);
+    unsigned long long alive = __ballot(live);
+    // run of this lane: [hl, he)
+    unsigned long long below = heads & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+    int hl = 63 - __builtin_clzll(below | 1ull);            // highest head <= lane
+    unsigned long long above = heads & ~((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+    int he = above ? __builtin_ctzll(above) : 64;
+    int nalive_to = 64 - __builtin_clzll(alive | 1ull);      // lanes [0, nalive_to) may be live
+    if (he > nalive_to) he = nalive_to;
+    int base = 0;
+    if (head) base = atomicAdd(&count[k], he - hl);
+    base = __shfl(base, hl);
+    if (live) {
+        key[p] = k;
+        slot[p] = base + (lane - hl);
+    }
+}
+
+// Per-tile totals of counts and of non-empty buckets.
+__global__ void __launch_bounds__(WG) k_scan_tiles(const int* __restrict__ count, long long NB, int* __restrict__ tiles) {
+    __shared__ int s_c[WG / 64], s_n[WG / 64];
+    long long b0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int c = 0, ne = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (b0 + i < NB) {
+            int v = count[b0 + i];
+            c += v;
+            ne += v > 0;
+        }
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o);
+        ne += __shfl_xor(ne, o);
+    }
+    if ((threadIdx.x & 63) == 0) { s_c[threadIdx.x >> 6] = c; s_n[threadIdx.x >> 6] = ne; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tc = 0, tn = 0;
+        for (int w = 0; w < WG / 64; ++w) { tc += s_c[w]; tn += s_n[w]; }
+        tiles[2 * blockIdx.x] = tc;
+        tiles[2 * blockIdx.x + 1] = tn;
+    }
+}
+
+// Exclusive block scan of (a, b) pairs held 4 per thread; returns thread's exclusive prefix.
+__device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int ia = a, ib = b;
+    for (int o = 1; o < 64; o <<= 1) {
+        int ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
+        if (lane >= o) { ia += ta; ib += tb; }
+    }
+    if (lane == 63) { s_a[w] = ia; s_b[w] = ib; }
+    __syncthreads();
+    int pa = 0, pb = 0;
+    for (int j = 0; j < w; ++j) { pa += s_a[j]; pb += s_b[j]; }
+    a = pa + ia - a;
+    b = pb + ib - b;
+}
+
+// Bucket starts, ordered non-empty bucket list, grid-block marks; clears count.
+__global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long long NB, int nb, const int* __restrict__ tiles,
+                                                   long long ntiles, int* __restrict__ start, int* __restrict__ alist,
+                                                   int* __restrict__ slotof, int* __restrict__ mark, int* __restrict__ misc) {
+    __shared__ int s_a[WG / 64], s_b[WG / 64], s_pa, s_pb;
+    // prefix of earlier tiles
+    int pa = 0, pb = 0;
+    for (long long t = threadIdx.x; t < blockIdx.x; t += WG) { pa += tiles[2 * t]; pb += tiles[2 * t + 1]; }
+    for (int o = 32; o > 0; o >>= 1) { pa += __shfl_xor(pa, o); pb += __shfl_xor(pb, o); }
+    if ((threadIdx.x & 63) == 0) { s_a[threadIdx.x >> 6] = pa; s_b[threadIdx.x >> 6] = pb; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int A = 0, B = 0;
+        for (int w = 0; w < WG / 64; ++w) { A += s_a[w]; B += s_b[w]; }
+        s_pa = A; s_pb = B;
+    }
+    __syncthreads();
+    const int tpa = s_pa, tpb = s_pb;
+    __syncthreads();
+    long long b0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v[4], a = 0, b = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        v[i] = b0 + i < NB ? count[b0 + i] : 0;
+        a += v[i];
+        b += v[i] > 0;
+    }
+    block_scan2(a, b, s_a, s_b);
+    a += tpa;
+    b += tpb;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        long long bk = b0 + i;
+        if (bk < NB) {
+            start[bk] = a;
+            count[bk] = 0;
+            slotof[bk] = v[i] > 0 ? b : -1;
+            if (v[i] > 0) {
+                alist[b] = (int)bk;
+                int bx = (int)(bk / ((long long)nb * nb)), by = (int)((bk / nb) % nb), bz = (int)(bk % nb);
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dz = -1; dz <= 1; ++dz) {
+                            int X = bx + dx, Y = by + dy, Z = bz + dz;
+                            if (X >= 0 && X < nb && Y >= 0 && Y < nb && Z >= 0 && Z < nb)
+                                mark[(X * nb + Y) * nb + Z] = 1;
+                        }
+                ++b;
+            }
+            a += v[i];
+        }
+    }
+    if (blockIdx.x == ntiles - 1 && threadIdx.x == WG - 1) {
+        start[NB] = a;
+        misc[0] = b;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) misc[1] = 0;
+}
+
+// Compact the marked grid blocks into blist (any order) and clear the marks.
+__global__ void __launch_bounds__(WG) k_mark_compact(int* __restrict__ mark, long long NB, int* __restrict__ blist,
+                                                     int* __restrict__ misc) {
+    const long long b = (long long)blockIdx.x * WG + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool mk = b < NB && mark[b] != 0;
+    if (mk) mark[b] = 0;
+    const unsigned long long bal = __ballot(mk);
+    if (!bal) return;
+    int base = 0;
+    if (lane == __builtin_ctzll(bal)) base = atomicAdd(&misc[1], __popcll(bal));
+    base = __shfl(base, __builtin_ctzll(bal));
+    if (mk) blist[base + __popcll(bal & ((1ull << lane) - 1))] = (int)b;
+}
+
+__global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __restrict__ key, const int* __restrict__ slot,
+                                                    const int* __restrict__ start, int* __restrict__ perm) {
+    long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    if (p < n) perm[start[key[p]] + slot[p]] = (int)p;
+}
+
+// Sort each bucket's particle ids ascending (makes perm a stable argsort of key).
+constexpr int SORT_CAP = 2048;
+__global__ void __launch_bounds__(WG) k_bin_stabilize(const int* __restrict__ start, const int* __restrict__ alist,
+                                                      const int* __restrict__ misc, int* __restrict__ perm,
+                                                      int* __restrict__ tmp) {
+    __shared__ int s[SORT_CAP];
+    const int nact = misc[0];
+    for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
+        const int bk = alist[ai];
+        const int s0 = start[bk], cnt = start[bk + 1] - s0;
+        if (cnt <= 1) continue;
+        if (cnt <= SORT_CAP) {
+            int n2 = 1;
+            while (n2 < cnt) n2 <<= 1;
+            for (int i = threadIdx.x; i < n2; i += WG) s[i] = i < cnt ? perm[s0 + i] : 0x7fffffff;
+            __syncthreads();
+            for (int k = 2; k <= n2; k <<= 1)
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = threadIdx.x; i < n2; i += WG) {
+                        int l = i ^ j;
+                        if (l > i) {
+                            int a = s[i], b = s[l];
+                            bool up = (i & k) == 0;
+                            if ((a > b) == up) { s[i] = b; s[l] = a; }
+                        }
+                    }
+                    __syncthreads();
+                }
+            for (int i = threadIdx.x; i < cnt; i += WG) perm[s0 + i] = s[i];
+            __syncthreads();
+        } else {
+            // bucket larger than the LDS sort: each id's rank is the number of
+            // smaller ids in the bucket; place through tmp, then copy back
+            for (int i = threadIdx.x; i < cnt; i += WG) {
+                const int e = perm[s0 + i];
+                int r = 0;
+                for (int j = 0; j < cnt; ++j) r += perm[s0 + j] < e;
+                tmp[s0 + r] = e;
+            }
+            __threadfence_block();
+            __syncthreads();
+            for (int i = threadIdx.x; i < cnt; i += WG) perm[s0 + i] = tmp[s0 + i];
+            __threadfence_block();
+            __syncthreads();
+        }
+    }
+}
+
+// ------------------------------------------------------------------ step kernels
+template <typename T> struct StepArgs {
+    // particles in
+    const T *x, *v, *F, *C, *d, *lap, *D_hist, *vol0, *mass, *mu, *lam, *sigma_c, *eta, *zeta, *l0, *residual, *A0,
+        *n_if;
+    const uint8_t *allow, *imask, *bc_id;
+    // particles out
+    T *ox, *ov, *oF, *oC, *olap, *od, *oD, *ophi, *ophs, *otau, *oJ;
+    // workspace
+    T *arena, *gv;
+    uint8_t* active;
+    const int *start, *alist, *slotof, *blist, *perm, *misc;
+    unsigned long long* stats;
+    // scalars
+    long long n;
+    int ng, nb;
+    T dx, inv_dx, dt, lim, pad_lo, pad_hi, fc_lo, fc_hi, dt_dmg, damp, stress_k, c_scale, lap_scale;
+    T dtg[3];
+    int use_fclamp, export_active, n_col, n_bc;
+    pmpm_collider col[PMPM_MAX_COLLIDERS];
+    T bcv[PMPM_MAX_VEL_BC][3];
+};
+
+// Persistent-bucket schedule: the non-empty bucket list is cut into 8
+// contiguous parts, one per XCD (work-groups are dealt to the 8 XCDs round
+// robin), so neighbouring buckets -- which share halo nodes -- run on the
+// same XCD and hit its L2.
+struct BucketIter {
+    int i, end, step;
+    __device__ BucketIter(int nact) {
+        const int G = gridDim.x, xcd = blockIdx.x & 7, lw = blockIdx.x >> 3, per = G >> 3;
+        const long long lo = (long long)nact * xcd / 8, hi = (long long)nact * (xcd + 1) / 8;
+        i = (int)lo + lw;
+        end = (int)hi;
+        step = per;
+    }
+};
+
+template <typename T> __device__ __forceinline__ M3<T> load_m3(const T* __restrict__ a, long long p) {
+    M3<T> r;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r.m[i][j] = a[9 * p + 3 * i + j];
+    return r;
+}
+template <typename T> __device__ __forceinline__ void store_m3(T* __restrict__ a, long long p, const M3<T>& r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) a[9 * p + 3 * i + j] = r.m[i][j];
+}
+
+// Stress, damage and degradation of one particle (anisompm.py:614-706).
+// Returns the degraded Kirchhoff stress; F is replaced by its clamped value
+// when f_clamp is on; d becomes the post-update damage.
+template <typename T, bool DAMAGE, bool EXACT_SIGMA>
+__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J) {
+    M3<T> Cm = mul_at(F, F);
+    T lc[3];
+    M3<T> V;
+    sym_eig3(Cm, lc, V);
+    T S[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S[i] = sqrt(maxp(lc[i], T(1e-12)));
+    M3<T> W = mul(F, V), Wh;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        T nrm = sqrt(W.m[0][j] * W.m[0][j] + W.m[1][j] * W.m[1][j] + W.m[2][j] * W.m[2][j]) + T(1e-12);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] / nrm;
+    }
+    M3<T> R = mul_bt(Wh, V);
+    if (A.use_fclamp) {
+        T sc[3], r[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            sc[i] = minp(maxp(S[i], A.fc_lo), A.fc_hi);
+            r[i] = sc[i] / maxp(S[i], T(1e-12));
+        }
+        M3<T> VD;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) VD.m[i][j] = V.m[i][j] * r[j];
+        F = mul(F, mul_bt(VD, V));
+#pragma unroll
+        for (int i = 0; i < 3; ++i) S[i] = sc[i];
+    }
+    J = maxp(S[0] * S[1] * S[2], T(1e-6));
+    const T mu = A.mu[p], lam = A.lam[p];
+    M3<T> FmR;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) FmR.m[i][j] = (T(2) * mu) * (F.m[i][j] - R.m[i][j]);
+    M3<T> tdev = mul_bt(FmR, F);
+    const T tvol = lam * (J - T(1)) * J;
+    const bool iface = A.imask[p] != 0;
+    T n0[3] = {T(0), T(0), T(0)};
+    if (iface) {
+        n0[0] = A.n_if[3 * p]; n0[1] = A.n_if[3 * p + 1]; n0[2] = A.n_if[3 * p + 2];
+    }
+
+    if (DAMAGE) {
+        const T sc = A.sigma_c[p];
+        T phi, tt2 = T(0);
+        if (EXACT_SIGMA) {
+            // Cauchy stress, its own Jacobi, tensile part (anisompm.py:650-655)
+            M3<T> sig;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) sig.m[i][j] = (tdev.m[i][j] + (i == j ? tvol : T(0))) / J;
+            T ev[3];
+            M3<T> Q;
+            sym_eig3(sig, ev, Q);
+            M3<T> QD;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) QD.m[i][j] = Q.m[i][j] * maxp(ev[j], T(0));
+            M3<T> sp = mul_bt(QD, Q);
+            M3<T> A0 = load_m3(A.A0, p);
+            M3<T> Am = mul_bt(mul(R, A0), R);
+            M3<T> M = mul(Am, sp);
+            T tr = T(0);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) tr += M.m[i][0] * M.m[0][i] + M.m[i][1] * M.m[1][i] + M.m[i][2] * M.m[2][i];
+            phi = tr / (sc * sc);
+            if (iface) {
+                T Sn[3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Sn[i] = sp.m[i][0] * n0[0] + sp.m[i][1] * n0[1] + sp.m[i][2] * n0[2];
+                T tn = Sn[0] * n0[0] + Sn[1] * n0[1] + Sn[2] * n0[2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { T u = Sn[i] - tn * n0[i]; tt2 += u * u; }
+            }
+        } else {
+            // F = R U with U = V diag(S) V^T, so tau = (R V) diag(2mu(S-1)S + tvol) (R V)^T:
+            // sigma's eigenvectors are the columns of W-hat and its eigenvalues are closed form.
+            T D[3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) D[i] = maxp(((T(2) * mu) * (S[i] - T(1)) * S[i] + tvol) / J, T(0));
+            // phi = tr(A s+ A s+) = sum_ij B_ij B_ji D_i D_j with B = V^T A0 V
+            M3<T> A0 = load_m3(A.A0, p);
+            M3<T> B = mul_at(V, mul(A0, V));
+            T tr = T(0);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) tr += B.m[i][j] * B.m[j][i] * (D[i] * D[j]);
+            phi = tr / (sc * sc);
+            if (iface) {
+                T u[3], Sn[3];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) u[j] = D[j] * (Wh.m[0][j] * n0[0] + Wh.m[1][j] * n0[1] + Wh.m[2][j] * n0[2]);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) Sn[i] = Wh.m[i][0] * u[0] + Wh.m[i][1] * u[1] + Wh.m[i][2] * u[2];
+                T tn = Sn[0] * n0[0] + Sn[1] * n0[1] + Sn[2] * n0[2];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) { T w = Sn[i] - tn * n0[i]; tt2 += w * w; }
+            }
+        }
+        T tt = sqrt(tt2) / sc;
+        A.ophi[p] = phi;
+        A.ophs[p] = tt * tt;
+        // history max of the drive, explicit phase-field update (anisompm.py:679-685)
+        const T Dh = maxp(A.D_hist[p], A.zeta[p] * relu(phi - T(1)));
+        A.oD[p] = Dh;
+        const T l0 = A.l0[p];
+        const T resist = d - (l0 * l0) * A.lap[p];
+        const T ddot = relu((T(1) - d) * Dh - resist);
+        T dn = d + (A.dt_dmg / A.eta[p]) * ddot;
+        dn = minp(maxp(dn, T(0)), T(1));
+        if (A.allow[p]) d = maxp(dn, d);
+        A.od[p] = d;
+    }
+
+    // degradation g(d), volumetric part only in tension (anisompm.py:687-691)
+    const T res = A.residual[p];
+    const T om = T(1) - d;
+    const T g = om * om * (T(1) - res) + res;
+    const T vdeg = J >= T(1) ? g : T(1);
+    M3<T> tau;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) tau.m[i][j] = g * tdev.m[i][j] + (i == j ? vdeg * tvol : T(0));
+    if (iface) {
+        // projection split by the deformed normal (anisompm.py:190-227, :697-703)
+        M3<T> tc = tdev;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tc.m[i][i] += tvol;
+        T m[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = F.m[i][0] * n0[0] + F.m[i][1] * n0[1] + F.m[i][2] * n0[2];
+        const T mn = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) + T(1e-9);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = m[i] / mn;
+        M3<T> P, Q;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                P.m[i][j] = m[i] * m[j];
+                Q.m[i][j] = (i == j ? T(1) : T(0)) - P.m[i][j];
+            }
+        M3<T> Pt = mul(P, tc), Qt = mul(Q, tc);
+        M3<T> tperp = mul(Pt, P), tpq = mul(Pt, Q), tqp = mul(Qt, P), tpar = mul(Qt, Q);
+        T tm[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tm[i] = tc.m[i][0] * m[0] + tc.m[i][1] * m[1] + tc.m[i][2] * m[2];
+        const T snn = m[0] * tm[0] + m[1] * tm[1] + m[2] * tm[2];
+        const T gI = snn > T(0) ? g : T(1);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tau.m[i][j] = tpar.m[i][j] + gI * tperp.m[i][j] + g * (tpq.m[i][j] + tqp.m[i][j]);
+    }
+    return tau;
+}
+
+template <typename T> __device__ __forceinline__ void lds_add(T* a, T v) { atomicAdd(a, v); }
+
+// Stress + damage + P2G (quadratic momentum, cubic damage) per bucket.
+template <typename T, bool DAMAGE, bool EXACT_SIGMA>
+__global__ void __launch_bounds__(WG) k_p2g(StepArgs<T> A) {
+    __shared__ T s_acc[6][AR3];      // m, mvx, mvy, mvz, sum w d, sum w
+    const int nact = A.misc[0];
+    const int nb = A.nb, ng = A.ng;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
+        for (int l = threadIdx.x; l < 6 * AR3; l += WG) (&s_acc[0][0])[l] = T(0);
+        __syncthreads();
+        const int s0 = A.start[bk], cnt = A.start[bk + 1] - s0;
+        for (int i = threadIdx.x; i < cnt; i += WG) {
+            const long long p = A.perm[s0 + i];
+            M3<T> F = load_m3(A.F, p);
+            T d = A.d[p], J;
+            M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J);
+            if (A.use_fclamp) store_m3(A.oF, p, F);
+            store_m3(A.otau, p, tau);
+            A.oJ[p] = J;
+            // affine momentum (anisompm.py:712-714)
+            const T mass = A.mass[p];
+            T sk = T(-1) * A.dt * A.vol0[p];
+            sk = sk * T(4);
+            sk = sk * A.inv_dx;
+            sk = sk * A.inv_dx;
+            M3<T> Cp = load_m3(A.C, p), af;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) af.m[a][b] = sk * tau.m[a][b] + mass * Cp.m[a][b];
+            T vv[3];
+            const int bc = A.bc_id ? A.bc_id[p] : 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) vv[a] = bc > 0 ? A.bcv[bc - 1][a] : A.v[3 * p + a];
+            T mv[3] = {mass * vv[0], mass * vv[1], mass * vv[2]};
+            T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx), scaled(A.x[3 * p + 2], A.inv_dx)};
+            int bq[3];
+            T fx[3], w[3][3];
+            quad_weights(Xp, ng, bq, fx, w);
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const int lx = clampi(bq[0] + ox, 0, ng - 1) - o0;
+#pragma unroll
+                for (int oy = 0; oy < 3; ++oy) {
+                    const int ly = clampi(bq[1] + oy, 0, ng - 1) - o1;
+                    const T wxy = w[ox][0] * w[oy][1];
+#pragma unroll
+                    for (int oz = 0; oz < 3; ++oz) {
+                        const int lz = clampi(bq[2] + oz, 0, ng - 1) - o2;
+                        const int l = (lx * AR + ly) * AR + lz;
+                        const T wk = wxy * w[oz][2];
+                        const T dp[3] = {(T(ox) - fx[0]) * A.dx, (T(oy) - fx[1]) * A.dx, (T(oz) - fx[2]) * A.dx};
+                        lds_add(&s_acc[0][l], wk * mass);
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+                            lds_add(&s_acc[1 + a][l],
+                                    wk * (mv[a] + (af.m[a][0] * dp[0] + af.m[a][1] * dp[1] + af.m[a][2] * dp[2])));
+                    }
+                }
+            }
+            // cubic scatter of the post-update damage (anisompm.py:768-776)
+            int bcb[3];
+            T wc[4][3], w2[4][3];
+            cubic_weights(Xp, ng, bcb, wc, w2);
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox) {
+                const int lx = clampi(bcb[0] + ox, 0, ng - 1) - o0;
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const int ly = clampi(bcb[1] + oy, 0, ng - 1) - o1;
+                    const T wxy = wc[ox][0] * wc[oy][1];
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const int lz = clampi(bcb[2] + oz, 0, ng - 1) - o2;
+                        const int l = (lx * AR + ly) * AR + lz;
+                        const T wk = wxy * wc[oz][2];
+                        lds_add(&s_acc[4][l], wk * d);
+                        lds_add(&s_acc[5][l], wk);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // store the arena to this bucket's slot: plain coalesced stores
+        T* dst = A.arena + (size_t)it.i * (6 * AR3);
+        for (int l = threadIdx.x; l < 6 * AR3; l += WG) dst[l] = (&s_acc[0][0])[l];
+        __syncthreads();
+    }
+}
+
+// Grid velocity update and colliders per marked 4x4x4 node block
+// (anisompm.py:726-746), cubic normalisation (:777-779).  A node n (per
+// axis n = 4q + r) lies in the arena [4b-1, 4b+5] of buckets b = q-1..q for
+// r <= 1, b = q for r = 2 and b = q..q+1 for r = 3; the node's mass,
+// momentum and cubic sums are the sum over those buckets' arenas.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
+    const int nwaves = gridDim.x * (WG / 64);
+    const int nblocks = A.misc[1];
+    const int nb = A.nb, ng = A.ng;
+    unsigned long long nq = 0, nc = 0;
+    for (int i = wave; i < nblocks; i += nwaves) {
+        const int blk = A.blist[i];
+        const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+        const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
+        int lo[3], hi[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int q = g[a] >> 2, r = g[a] & 3;
+            lo[a] = r <= 1 ? q - 1 : q;
+            hi[a] = r == 3 ? q + 1 : q;
+            lo[a] = lo[a] < 0 ? 0 : lo[a];
+            hi[a] = hi[a] > nb - 1 ? nb - 1 : hi[a];
+        }
+        T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+        for (int u = lo[0]; u <= hi[0]; ++u)
+            for (int v = lo[1]; v <= hi[1]; ++v)
+                for (int w = lo[2]; w <= hi[2]; ++w) {
+                    const int slot = A.slotof[(u * nb + v) * nb + w];
+                    if (slot < 0) continue;
+                    const int l = ((g[0] - (BLK * u - 1)) * AR + (g[1] - (BLK * v - 1))) * AR + (g[2] - (BLK * w - 1));
+                    const T* src = A.arena + (size_t)slot * (6 * AR3) + l;
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
+                }
+        const int gx = g[0], gy = g[1], gz = g[2];
+        const long long node = ((long long)gx * ng + gy) * ng + gz;
+        const T m = acc[0], mvx = acc[1], mvy = acc[2], mvz = acc[3], dsum = acc[4], wsum = acc[5];
+        {
+            const bool act = m > T(1e-12);
+            T v[3] = {T(0), T(0), T(0)};
+            if (act) {
+                const T inv = T(1) / m;
+                v[0] = mvx * inv + A.dtg[0];
+                v[1] = mvy * inv + A.dtg[1];
+                v[2] = mvz * inv + A.dtg[2];
+                if (A.damp != T(1)) { v[0] = v[0] * A.damp; v[1] = v[1] * A.damp; v[2] = v[2] * A.damp; }
+                const T pos[3] = {T(gx) * A.dx, T(gy) * A.dx, T(gz) * A.dx};
+                for (int k = 0; k < A.n_col; ++k) {
+                    const pmpm_collider& cl = A.col[k];
+                    const T p0[3] = {T(cl.p0[0]), T(cl.p0[1]), T(cl.p0[2])};
+                    const T p1[3] = {T(cl.p1[0]), T(cl.p1[1]), T(cl.p1[2])};
+                    const T vc[3] = {(p1[0] - p0[0]) / T(1e-4), (p1[1] - p0[1]) / T(1e-4), (p1[2] - p0[2]) / T(1e-4)};
+                    T n[3];
+                    bool inside;
+                    if (cl.kind == PMPM_COLLIDER_SPHERE) {
+                        const T r[3] = {pos[0] - p0[0], pos[1] - p0[1], pos[2] - p0[2]};
+                        const T dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+                        inside = dist < T(cl.radius);
+                        const T den = dist + T(1e-9);
+                        n[0] = r[0] / den; n[1] = r[1] / den; n[2] = r[2] / den;
+                    } else {
+                        n[0] = T(cl.normal[0]); n[1] = T(cl.normal[1]); n[2] = T(cl.normal[2]);
+                        const T sd = (pos[0] - p0[0]) * n[0] + (pos[1] - p0[1]) * n[1] + (pos[2] - p0[2]) * n[2];
+                        inside = sd < T(0);
+                    }
+                    if (inside) collider_resolve(v, vc, n, cl.mode, T(cl.friction));
+                }
+            }
+            const bool cact = wsum > T(1e-12);
+            T* g = A.gv + 4 * node;
+            g[0] = v[0]; g[1] = v[1]; g[2] = v[2];
+            g[3] = cact ? dsum / wsum : T(0);
+            if (A.export_active) A.active[node] = act ? 1 : 0;
+            nq += act;
+            nc += cact;
+        }
+    }
+    // per-wave totals of active nodes
+    for (int o = 32; o > 0; o >>= 1) {
+        nq += __shfl_xor(nq, o);
+        nc += __shfl_xor(nc, o);
+    }
+    if (lane == 0 && (nq | nc)) {
+        atomicAdd(&A.stats[0], nq);
+        atomicAdd(&A.stats[1], nc);
+    }
+}
+
+// G2P + cubic Laplacian + advection/F update/clamp per bucket (anisompm.py:748-803).
+template <typename T>
+__global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
+    __shared__ T s_g[4][AR3];
+    const int nact = A.misc[0];
+    const int nb = A.nb, ng = A.ng;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
+        for (int l = threadIdx.x; l < AR3; l += WG) {
+            const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
+            const int gx = o0 + lx, gy = o1 + ly, gz = o2 + lz;
+            if (gx < 0 || gy < 0 || gz < 0 || gx >= ng || gy >= ng || gz >= ng) {
+                s_g[0][l] = T(0); s_g[1][l] = T(0); s_g[2][l] = T(0); s_g[3][l] = T(0);
+            } else {
+                const T* g = A.gv + 4 * (((long long)gx * ng + gy) * ng + gz);
+                s_g[0][l] = g[0]; s_g[1][l] = g[1]; s_g[2][l] = g[2]; s_g[3][l] = g[3];
+            }
+        }
+        __syncthreads();
+        const int s0 = A.start[bk], cnt = A.start[bk + 1] - s0;
+        for (int i = threadIdx.x; i < cnt; i += WG) {
+            const long long p = A.perm[s0 + i];
+            T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+            T Xp[3] = {scaled(xp[0], A.inv_dx), scaled(xp[1], A.inv_dx), scaled(xp[2], A.inv_dx)};
+            int bq[3];
+            T fx[3], w[3][3];
+            quad_weights(Xp, ng, bq, fx, w);
+            T vn[3] = {T(0), T(0), T(0)};
+            M3<T> Cn;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Cn.m[a][b] = T(0);
+#pragma unroll
+            for (int ox = 0; ox < 3; ++ox) {
+                const int lx = clampi(bq[0] + ox, 0, ng - 1) - o0;
+#pragma unroll
+                for (int oy = 0; oy < 3; ++oy) {
+                    const int ly = clampi(bq[1] + oy, 0, ng - 1) - o1;
+                    const T wxy = w[ox][0] * w[oy][1];
+#pragma unroll
+                    for (int oz = 0; oz < 3; ++oz) {
+                        const int lz = clampi(bq[2] + oz, 0, ng - 1) - o2;
+                        const int l = (lx * AR + ly) * AR + lz;
+                        const T wk = wxy * w[oz][2];
+                        const T g[3] = {s_g[0][l], s_g[1][l], s_g[2][l]};
+                        const T dp[3] = {(T(ox) - fx[0]) * A.dx, (T(oy) - fx[1]) * A.dx, (T(oz) - fx[2]) * A.dx};
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            vn[a] = vn[a] + wk * g[a];
+#pragma unroll
+                            for (int b = 0; b < 3; ++b) Cn.m[a][b] = Cn.m[a][b] + wk * (g[a] * dp[b]);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Cn.m[a][b] = Cn.m[a][b] * A.c_scale;
+            int bcb[3];
+            T wc[4][3], w2[4][3];
+            cubic_weights(Xp, ng, bcb, wc, w2);
+            T lap = T(0);
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox) {
+                const int lx = clampi(bcb[0] + ox, 0, ng - 1) - o0;
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const int ly = clampi(bcb[1] + oy, 0, ng - 1) - o1;
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const int lz = clampi(bcb[2] + oz, 0, ng - 1) - o2;
+                        const int l = (lx * AR + ly) * AR + lz;
+                        const T lw = A.lap_scale * (w2[ox][0] * wc[oy][1] * wc[oz][2] + wc[ox][0] * w2[oy][1] * wc[oz][2] +
+                                                    wc[ox][0] * wc[oy][1] * w2[oz][2]);
+                        lap = lap + lw * s_g[3][l];
+                    }
+                }
+            }
+            const M3<T> F = load_m3(A.use_fclamp ? (const T*)A.oF : A.F, p);
+            M3<T> Ip;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) Ip.m[a][b] = (a == b ? T(1) : T(0)) + A.dt * Cn.m[a][b];
+            store_m3(A.oF, p, mul(Ip, F));
+            store_m3(A.oC, p, Cn);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                A.ov[3 * p + a] = vn[a];
+                const T xn = xp[a] + A.dt * vn[a];
+                A.ox[3 * p + a] = minp(maxp(xn, A.pad_lo), A.pad_hi);
+            }
+            A.olap[p] = lap;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------ test kernels
+template <typename T>
+__global__ void __launch_bounds__(WG) k_sym_eig(const T* __restrict__ A, long long n, T* __restrict__ ev, T* __restrict__ V) {
+    long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    if (p >= n) return;
+    M3<T> a = load_m3(A, p), Q;
+    T e[3];
+    sym_eig3(a, e, Q);
+    ev[3 * p] = e[0]; ev[3 * p + 1] = e[1]; ev[3 * p + 2] = e[2];
+    store_m3(V, p, Q);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(WG) k_stencil(const T* __restrict__ x, long long n, int ng, T inv_dx, int* bq_o, int* i27, int* bc_o,
+                                                int* i64, T* w_o, T* wc_o, T* w2_o) {
+    long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    if (p >= n) return;
+    T Xp[3] = {scaled(x[3 * p], inv_dx), scaled(x[3 * p + 1], inv_dx), scaled(x[3 * p + 2], inv_dx)};
+    int bq[3], bcb[3];
+    T fx[3], w[3][3], wc[4][3], w2[4][3];
+    quad_weights(Xp, ng, bq, fx, w);
+    cubic_weights(Xp, ng, bcb, wc, w2);
+    for (int a = 0; a < 3; ++a) {
+        // report the reference's unclamped base cells (the kernels clamp internally)
+        const T lim = T(1 << 30);
+        if (bq_o) bq_o[3 * p + a] = (int)minp(maxp(floor(sub_exact(Xp[a], T(0.5))), -lim), lim);
+        if (bc_o) bc_o[3 * p + a] = (int)minp(maxp(floor(Xp[a]), -lim), lim) - 1;
+        for (int k = 0; k < 3; ++k)
+            if (w_o) w_o[9 * p + 3 * k + a] = w[k][a];
+        for (int k = 0; k < 4; ++k) {
+            if (wc_o) wc_o[12 * p + 3 * k + a] = wc[k][a];
+            if (w2_o) w2_o[12 * p + 3 * k + a] = w2[k][a];
+        }
+    }
+    if (i27)
+        for (int k = 0; k < 27; ++k) i27[27 * p + k] = node_lin(bq[0] + k / 9, bq[1] + (k / 3) % 3, bq[2] + k % 3, ng);
+    if (i64)
+        for (int k = 0; k < 64; ++k) i64[64 * p + k] = node_lin(bcb[0] + k / 16, bcb[1] + (k / 4) % 4, bcb[2] + k % 4, ng);
+}
+
+// ------------------------------------------------------------------ host side
+inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
+
+template <typename T>
+int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const Layout& L, char* ws, hipStream_t st) {
+    const int nb = nblk(ng);
+    int* count = (int*)(ws + L.count);
+    int* start = (int*)(ws + L.start);
+    int* tiles = (int*)(ws + L.tiles);
+    int* alist = (int*)(ws + L.alist);
+    int* slotof = (int*)(ws + L.slotof);
+    int* mark = (int*)(ws + L.mark);
+    int* key = (int*)(ws + L.key);
+    int* slot = (int*)(ws + L.slot);
+    int* perm = (int*)(ws + L.perm);
+    int* misc = (int*)(ws + L.misc);
+    TimedScope ts(0, st);
+    if (n > 0) hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, count, key, slot);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NB, tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NB, nb, tiles, L.ntiles, start, alist,
+                       slotof, mark, misc);
+    if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm);
+    if (stable && n > 0)
+        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(WG), 0, st, start, alist, misc, perm, slot);
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out,
+                                           const Layout& L, char* ws) {
+    StepArgs<T> A;
+    memset(&A, 0, sizeof A);
+    A.x = (const T*)in->x; A.v = (const T*)in->v; A.F = (const T*)in->F; A.C = (const T*)in->C;
+    A.d = (const T*)in->d; A.lap = (const T*)in->lap; A.D_hist = (const T*)in->D_hist;
+    A.vol0 = (const T*)in->vol0; A.mass = (const T*)in->mass; A.mu = (const T*)in->mu; A.lam = (const T*)in->lam;
+    A.sigma_c = (const T*)in->sigma_c; A.eta = (const T*)in->eta; A.zeta = (const T*)in->zeta; A.l0 = (const T*)in->l0;
+    A.residual = (const T*)in->residual; A.A0 = (const T*)in->A0; A.n_if = (const T*)in->n_interface;
+    A.allow = in->allow; A.imask = in->interface_mask; A.bc_id = in->bc_id;
+    A.ox = (T*)out->x; A.ov = (T*)out->v; A.oF = (T*)out->F; A.oC = (T*)out->C; A.olap = (T*)out->lap;
+    A.od = (T*)out->d; A.oD = (T*)out->D_hist; A.ophi = (T*)out->phi; A.ophs = (T*)out->phi_shear;
+    A.otau = (T*)out->tau_last; A.oJ = (T*)out->J_last;
+    A.arena = (T*)(ws + L.arena); A.gv = (T*)(ws + L.gv);
+    A.active = (uint8_t*)(ws + L.active);
+    A.start = (const int*)(ws + L.start); A.alist = (const int*)(ws + L.alist); A.perm = (const int*)(ws + L.perm);
+    A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
+    A.misc = (const int*)(ws + L.misc);
+    A.stats = (unsigned long long*)(ws + L.misc + 16);
+    A.n = in->n;
+    A.ng = p->n_grid; A.nb = nblk(p->n_grid);
+    A.dx = T(p->dx); A.inv_dx = T(p->inv_dx); A.dt = T(p->dt); A.lim = T(p->grid_lim);
+    A.pad_lo = T(2.0 * p->dx); A.pad_hi = T(p->grid_lim - 2.0 * p->dx);
+    A.fc_lo = T(p->fclamp_lo); A.fc_hi = T(p->fclamp_hi); A.use_fclamp = p->use_fclamp;
+    A.dt_dmg = T(p->dt_damage); A.damp = T(p->grid_damp);
+    A.c_scale = T(4.0 * p->inv_dx * p->inv_dx);
+    A.lap_scale = T(p->inv_dx * p->inv_dx);
+    for (int a = 0; a < 3; ++a) A.dtg[a] = T(p->dt) * T(p->gravity[a]);
+    A.export_active = p->export_active;
+    A.n_col = p->n_colliders;
+    for (int k = 0; k < p->n_colliders; ++k) A.col[k] = p->colliders[k];
+    A.n_bc = p->n_vel_bc;
+    for (int k = 0; k < p->n_vel_bc; ++k)
+        for (int a = 0; a < 3; ++a) A.bcv[k][a] = T(p->vel_bc[k][a]);
+    return A;
+}
+
+int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, size_t ws_bytes,
+               int64_t& max_n_out) {
+    if (!p || !in || !out) return fail(PMPM_EARG, "%snull argument", "");
+    if (p->dtype != PMPM_F32 && p->dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", p->dtype);
+    if (p->n_grid < 4 || p->n_grid > 1024) return fail(PMPM_EARG, "%sn_grid out of range: %lld", "", p->n_grid);
+    if (in->n < 0 || in->n > 0x7fffffffLL) return fail(PMPM_EARG, "%sparticle count out of range: %lld", "", in->n);
+    if (p->n_colliders < 0 || p->n_colliders > PMPM_MAX_COLLIDERS) return fail(PMPM_EARG, "%stoo many colliders: %lld", "", p->n_colliders);
+    if (p->n_vel_bc < 0 || p->n_vel_bc > PMPM_MAX_VEL_BC) return fail(PMPM_EARG, "%stoo many velocity BCs: %lld", "", p->n_vel_bc);
+    Layout L(in->n, p->n_grid, p->dtype);
+    if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
+    if (in->n > 0) {
+        const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, in->vol0, in->mass, in->mu, in->lam,
+                              in->residual, in->interface_mask, out->x, out->v, out->F, out->C, out->lap, out->tau_last,
+                              out->J_last};
+        for (const void* q : need)
+            if (!q) return fail(PMPM_EARG, "%smissing particle array", "");
+        if (in->n_interface == nullptr) return fail(PMPM_EARG, "%smissing n_interface", "");
+        if (p->do_damage) {
+            const void* dm[] = {in->D_hist, in->sigma_c, in->eta, in->zeta, in->l0, in->A0, in->allow,
+                                out->d, out->D_hist, out->phi, out->phi_shear};
+            for (const void* q : dm)
+                if (!q) return fail(PMPM_EARG, "%smissing damage array", "");
+        }
+        if (out->x == in->x || out->F == in->F || out->v == in->v || out->C == in->C)
+            return fail(PMPM_EARG, "%soutputs must not alias inputs", "");
+    }
+    max_n_out = in->n;
+    return PMPM_OK;
+}
+
+template <typename T>
+int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
+               int exact_sigma) {
+    Layout L(in->n, p->n_grid, p->dtype);
+    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, L, ws, st);
+    if (rc) return rc;
+    StepArgs<T> A = make_args<T>(p, in, out, L, ws);
+    {
+        TimedScope ts(0, st);
+        hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
+                           (int*)(ws + L.blist), (int*)(ws + L.misc));
+    }
+    HIPCHK(hipMemsetAsync(ws + L.misc + 16, 0, 16, st));
+    if (p->export_active) HIPCHK(hipMemsetAsync(ws + L.active, 0, (size_t)L.G, st));
+    if (in->n > 0) {
+        TimedScope ts(1, st);
+        if (p->do_damage) {
+            if (exact_sigma) hipLaunchKernelGGL((k_p2g<T, true, true>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
+            else hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
+        } else {
+            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
+        }
+    }
+    {
+        TimedScope ts(2, st);
+        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WAVES / (WG / 64)), dim3(WG), 0, st, A);
+    }
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+template <typename T>
+int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st) {
+    Layout L(in->n, p->n_grid, p->dtype);
+    StepArgs<T> A = make_args<T>(p, in, out, L, ws);
+    if (in->n > 0) {
+        TimedScope ts(3, st);
+        hipLaunchKernelGGL(k_g2p<T>, dim3(PERSIST_WG), dim3(WG), 0, st, A);
+    }
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* pmpm_last_error(void) { return g_err; }
+const char* pmpm_version(void) { return "pmpm 0.1 gfx950"; }
+
+int pmpm_abi_sizes(size_t* sizes, int32_t n) {
+    const size_t all[4] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
+                           sizeof(pmpm_particles_out)};
+    int k = 0;
+    for (; k < n && k < 4; ++k) sizes[k] = all[k];
+    return k;
+}
+
+size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype) {
+    if (n_grid < 1 || max_n < 0) return 0;
+    return Layout(max_n, n_grid, dtype).total;
+}
+
+int pmpm_workspace_init(void* ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t dtype, void* stream) {
+    if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
+    Layout L(max_n, n_grid, dtype);
+    if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
+    HIPCHK(hipMemsetAsync(ws, 0, L.total, (hipStream_t)stream));
+    return PMPM_OK;
+}
+
+int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t dtype, size_t* gv_offset, size_t* active_offset) {
+    Layout L(max_n, n_grid, dtype);
+    if (gv_offset) *gv_offset = L.gv;
+    if (active_offset) *active_offset = L.active;
+    return PMPM_OK;
+}
+
+int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t dtype, int64_t* n_active_q, int64_t* n_active_c,
+                         void* stream) {
+    Layout L(max_n, n_grid, dtype);
+    unsigned long long h[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(h, (const char*)ws + L.misc + 16, 16, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    if (n_active_q) *n_active_q = (int64_t)h[0];
+    if (n_active_c) *n_active_c = (int64_t)h[1];
+    return PMPM_OK;
+}
+
+int pmpm_substep_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
+                       size_t ws_bytes, void* stream) {
+    int64_t n;
+    int rc = check_step(p, in, out, ws_bytes, n);
+    if (rc) return rc;
+    if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
+    return p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, (hipStream_t)stream, p->exact_sigma_eig)
+                                : step_begin<float>(p, in, out, (char*)ws, (hipStream_t)stream, p->exact_sigma_eig);
+}
+
+int pmpm_substep_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
+                     size_t ws_bytes, void* stream) {
+    int64_t n;
+    int rc = check_step(p, in, out, ws_bytes, n);
+    if (rc) return rc;
+    return p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, (hipStream_t)stream)
+                                : step_end<float>(p, in, out, (char*)ws, (hipStream_t)stream);
+}
+
+int pmpm_substep(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
+                 size_t ws_bytes, void* stream) {
+    int rc = pmpm_substep_begin(p, in, out, ws, ws_bytes, stream);
+    if (rc) return rc;
+    return pmpm_substep_end(p, in, out, ws, ws_bytes, stream);
+}
+
+int pmpm_timing(int32_t enable) {
+    g_timing.reset();
+    g_timing.on = enable != 0;
+    return PMPM_OK;
+}
+
+int pmpm_timing_read(double* total_ms, int64_t* launches, int32_t n) {
+    for (int g = 0; g < NTIMED && g < n; ++g) {
+        double tot = 0;
+        auto& v = g_timing.ev[g];
+        for (size_t i = 0; i + 1 < v.size(); i += 2) {
+            HIPCHK(hipEventSynchronize(v[i + 1]));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, v[i], v[i + 1]));
+            tot += ms;
+        }
+        if (total_ms) total_ms[g] = tot;
+        if (launches) launches[g] = (int64_t)(v.size() / 2);
+    }
+    return PMPM_OK;
+}
+
+int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t stable, int32_t* key, int32_t* perm,
+             int32_t* bucket_start, void* ws, size_t ws_bytes, void* stream) {
+    if (!ws || (n > 0 && (!x || !perm))) return fail(PMPM_EARG, "%snull argument", "");
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
+    if (n_grid < 4 || n_grid > 1024 || n < 0 || n > 0x7fffffffLL) return fail(PMPM_EARG, "%sbad size %lld", "", n);
+    Layout L(n, n_grid, dtype);
+    if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
+    hipStream_t st = (hipStream_t)stream;
+    int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, L, (char*)ws, st)
+                               : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, L, (char*)ws, st);
+    if (rc) return rc;
+    // the grid-block marks of a bare binning call are not consumed by a grid pass
+    HIPCHK(hipMemsetAsync((char*)ws + L.mark, 0, (size_t)L.NB * 4, st));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(perm, (char*)ws + L.perm, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+        if (key) HIPCHK(hipMemcpyAsync(key, (char*)ws + L.key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    if (bucket_start) HIPCHK(hipMemcpyAsync(bucket_start, (char*)ws + L.start, (size_t)(L.NB + 1) * 4, hipMemcpyDeviceToDevice, st));
+    return PMPM_OK;
+}
+
+int pmpm_stencil(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t* base_q, int32_t* idx27,
+                 int32_t* base_c, int32_t* idx64, void* w, void* wc, void* wc2, void* stream) {
+    if (n < 0 || (n > 0 && !x)) return fail(PMPM_EARG, "%snull argument", "");
+    if (n == 0) return PMPM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == PMPM_F64)
+        hipLaunchKernelGGL(k_stencil<double>, dim3(nblocks(n)), dim3(WG), 0, st, (const double*)x, (long long)n, n_grid, inv_dx,
+                           base_q, idx27, base_c, idx64, (double*)w, (double*)wc, (double*)wc2);
+    else
+        hipLaunchKernelGGL(k_stencil<float>, dim3(nblocks(n)), dim3(WG), 0, st, (const float*)x, (long long)n, n_grid,
+                           (float)inv_dx, base_q, idx27, base_c, idx64, (float*)w, (float*)wc, (float*)wc2);
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+int pmpm_sym_eig(const void* A, int32_t dtype, int64_t n, void* evals, void* V, void* stream) {
+    if (n < 0 || (n > 0 && (!A || !evals || !V))) return fail(PMPM_EARG, "%snull argument", "");
+    if (n == 0) return PMPM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == PMPM_F64)
+        hipLaunchKernelGGL(k_sym_eig<double>, dim3(nblocks(n)), dim3(WG), 0, st, (const double*)A, (long long)n, (double*)evals,
+                           (double*)V);
+    else
+        hipLaunchKernelGGL(k_sym_eig<float>, dim3(nblocks(n)), dim3(WG), 0, st, (const float*)A, (long long)n, (float*)evals,
+                           (float*)V);
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,296 @@
+"""``AnisoMPM``: the reference's solver class with its step on MI355X kernels.
+
+Drop-in for ``AnisoMPM`` of anisompm_py/src/anisompm.py:303-807: the same
+constructor keywords, the same per-particle state tensors as public
+attributes (same names, shapes, dtypes, row-major layout), the same setup
+methods, ``colliders`` / ``particle_bc`` lists and ``substep`` /
+``run_frame``.  ``substep`` hands the tensors' device pointers to
+``libpmpm.so`` (include/pmpm.h) and rebinds the state attributes to the new
+tensors it wrote, as the reference's step rebinds them.
+
+Differences a caller can observe:
+* only float32/float64 on a ROCm device (``device="cuda:N"``);
+* ``particle_bc`` entries that are plain callables run on the host before
+  the step, exactly as in the reference; :class:`VelocityBC` entries are
+  applied inside the transfer kernel instead;
+* colliders made by this package's factories run on the device; any other
+  callable is applied between the grid update and G2P on the host;
+* the Cauchy-stress eigenpairs come from the polar stretch's eigenvectors
+  (closed form) unless ``exact_sigma_eig=True`` asks for the reference's
+  second Jacobi.  The two agree to float rounding.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from . import _lib
+from .boundaries import HalfspaceCollider, SphereCollider, VelocityBC
+
+_STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
+_FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "residual", "A0", "n_interface")
+
+
+class AnisoMPM:
+    def __init__(self, n_grid: int = 96, grid_lim: float = 1.0, dt: float = 1e-4, gravity=(0.0, -9.8, 0.0),
+                 grid_damp: float = 1.0, f_clamp=None, damage_every: int = 1, device: str = "cuda:0",
+                 dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = False):
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("the native AnisoMPM step runs on a ROCm GPU (device='cuda:N')")
+        self.dtype = dtype
+        self._code = _lib.dtype_code(dtype)
+        self.n_grid = int(n_grid)
+        self.grid_lim = float(grid_lim)
+        self.dx = grid_lim / n_grid
+        self.inv_dx = 1.0 / self.dx
+        self.dt = float(dt)
+        self.gravity = torch.tensor(gravity, device=self.device, dtype=dtype)
+        self._gravity_host = [float(g) for g in gravity]
+        self.grid_damp = float(grid_damp)
+        self.f_clamp = f_clamp
+        self.damage_every = max(1, int(damage_every))
+        self.exact_sigma_eig = bool(exact_sigma_eig)
+        self.stable_sort = bool(stable_sort)
+        self._step = 0
+        r3, r4 = torch.arange(3, device=self.device), torch.arange(4, device=self.device)
+        self.offsets = torch.stack([t.reshape(-1) for t in torch.meshgrid(r3, r3, r3, indexing="ij")], 1)
+        self.offsets4 = torch.stack([t.reshape(-1) for t in torch.meshgrid(r4, r4, r4, indexing="ij")], 1)
+        self.colliders = []
+        self.particle_bc = []
+        self.n = 0
+        self._ws = None
+        self._ws_n = -1
+        self._allow_key = None
+        self._allow_any = False
+        self._bc_key = None
+        self._bc_id = None
+        _lib.lib()
+
+    # ------------------------------------------------------------------ setup
+    def add_object(self, x, vol, rho: float = 500.0, E: float = 1.0e4, nu: float = 0.4, fibers=None,
+                   alpha: float = -1.0, percentage: float = 0.15, l0_scale: float = 0.5, eta: float = 0.01,
+                   zeta: float = 1.0, residual: float = 0.001, allow_damage: bool = True):
+        """Append particles (anisompm.py:350-427): Lame parameters from (E, nu),
+        A0 = I + alpha a0 a0^T from unit fibers, sigma_c = max(percentage,1e-8)*E."""
+        dev, dt = self.device, self.dtype
+        x = torch.as_tensor(x).to(dev, dt).contiguous()
+        vol = torch.as_tensor(vol).to(dev, dt).contiguous()
+        n = x.shape[0]
+        mu = E / (2.0 * (1.0 + nu))
+        lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        full = lambda val: torch.full((n,), float(val), device=dev, dtype=dt)
+        eye = torch.eye(3, device=dev, dtype=dt)
+        if fibers is None or alpha == 0.0:
+            a0 = torch.zeros(n, 3, device=dev, dtype=dt)
+            al = torch.zeros(n, device=dev, dtype=dt)
+        else:
+            f = torch.as_tensor(fibers).to(dev, dt)
+            a0 = f / (f.norm(dim=1, keepdim=True) + 1e-12)
+            al = full(alpha)
+        part = {
+            "x": x, "v": torch.zeros_like(x), "F": eye.expand(n, 3, 3).contiguous(),
+            "C": torch.zeros(n, 3, 3, device=dev, dtype=dt), "vol0": vol.clone(), "mass": vol * rho,
+            "mu": full(mu), "lam": full(lam), "d": full(0.0), "lap": full(0.0), "D_hist": full(0.0),
+            "n_interface": torch.zeros(n, 3, device=dev, dtype=dt),
+            "interface_mask": torch.zeros(n, device=dev, dtype=torch.bool),
+            "a0": a0, "alpha": al, "A0": eye + al.view(-1, 1, 1) * (a0.unsqueeze(2) * a0.unsqueeze(1)),
+            "sigma_c": full(max(percentage, 1e-8) * E), "eta": full(eta), "zeta": full(zeta),
+            "l0": full(l0_scale * self.dx), "residual": full(residual),
+            "allow": torch.full((n,), bool(allow_damage), device=dev, dtype=torch.bool),
+        }
+        for k, t in part.items():
+            setattr(self, k, t if self.n == 0 else torch.cat([getattr(self, k), t], 0))
+        self.n = self.x.shape[0]
+        return self
+
+    def set_radial_fibers(self, center, zero_dim: int = 1, alpha: float = -1.0):
+        """Fibers pointing away from ``center`` in the plane normal to axis
+        ``zero_dim``, and A0 rebuilt from them (anisompm.py:465-484)."""
+        r = self.x - torch.as_tensor(center, device=self.device, dtype=self.dtype)
+        r[:, zero_dim] = 0.0
+        nrm = r.norm(dim=1, keepdim=True)
+        self.a0 = torch.where(nrm > 1e-9, r / (nrm + 1e-12), torch.zeros_like(r))
+        self.alpha = torch.full((self.n,), float(alpha), device=self.device, dtype=self.dtype)
+        eye = torch.eye(3, device=self.device, dtype=self.dtype)
+        self.A0 = eye + self.alpha.view(-1, 1, 1) * (self.a0.unsqueeze(2) * self.a0.unsqueeze(1))
+        return self
+
+    def set_structural_tensor(self, mask, A0):
+        """Overwrite A0 on the masked particles with one (3,3) tensor, or per
+        particle from an (N,3,3) or (n_masked,3,3) array (anisompm.py:486-509)."""
+        mask = torch.as_tensor(mask, device=self.device).bool()
+        A = torch.as_tensor(A0, device=self.device, dtype=self.dtype)
+        k = int(mask.sum())
+        if A.dim() == 2:
+            A = A.unsqueeze(0).expand(k, 3, 3)
+        if A.shape[0] == k:
+            full = self.A0.clone()
+            full[mask] = A
+            A = full
+        self.A0 = torch.where(mask.view(-1, 1, 1), A, self.A0)
+        return self
+
+    def set_interface_normals(self, mask, normals):
+        """Register unit interface normals on the masked particles and switch
+        them to the directional stress split (anisompm.py:511-529)."""
+        mask = torch.as_tensor(mask, device=self.device).bool()
+        nrm = torch.as_tensor(normals, device=self.device, dtype=self.dtype)
+        if nrm.shape[0] == int(mask.sum()):
+            full = torch.zeros(self.n, 3, device=self.device, dtype=self.dtype)
+            full[mask] = nrm
+            nrm = full
+        nrm = nrm / (nrm.norm(dim=1, keepdim=True) + 1e-12)
+        self.n_interface = torch.where(mask.view(-1, 1), nrm, self.n_interface)
+        self.interface_mask = self.interface_mask | mask
+        return self
+
+    def set_normal_isolation(self, mask, normals):
+        """Interface normals plus A0 = n n^T on the same particles (anisompm.py:531-541)."""
+        self.set_interface_normals(mask, normals)
+        mask = torch.as_tensor(mask, device=self.device).bool()
+        nn = self.n_interface.unsqueeze(2) * self.n_interface.unsqueeze(1)
+        return self.set_structural_tensor(mask, nn[mask])
+
+    # ------------------------------------------------------------------ step
+    def _workspace(self):
+        if self._ws is None or self._ws_n != self.n:
+            L = _lib.lib()
+            nbytes = L.pmpm_workspace_bytes(self.n, self.n_grid, self._code)
+            self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            self._ws_n = self.n
+            _lib.check(L.pmpm_workspace_init(self._ws.data_ptr(), nbytes, self.n, self.n_grid, self._code,
+                                             _lib.stream_ptr(self.device)), "pmpm_workspace_init")
+        return self._ws
+
+    def _any_allow(self):
+        """allow.any(), recomputed (one host sync) only when ``allow`` changed."""
+        key = (id(self.allow), self.allow.data_ptr(), self.allow._version, self.allow.shape[0])
+        if key != self._allow_key:
+            self._allow_any = bool(self.allow.any())
+            self._allow_key = key
+        return self._allow_any
+
+    def _tensor(self, name, dtype=None):
+        t = getattr(self, name)
+        want = dtype or self.dtype
+        if t.device != self.device or t.dtype != want or not t.is_contiguous():
+            t = t.to(self.device, want).contiguous()
+            setattr(self, name, t)
+        return t
+
+    def _velocity_bcs(self, time):
+        """Fast path when every particle BC is a VelocityBC: a cached per-
+        particle slot id and this step's velocity table."""
+        bcs = self.particle_bc
+        if not bcs or not all(isinstance(b, VelocityBC) for b in bcs):
+            return None, []
+        if len(bcs) > _lib.MAX_VEL_BC:
+            raise ValueError(f"at most {_lib.MAX_VEL_BC} VelocityBC entries")
+        key = tuple((id(b), id(b.mask), getattr(b.mask, "_version", 0)) for b in bcs) + (self.n,)
+        if key != self._bc_key:
+            ids = torch.zeros(self.n, dtype=torch.uint8, device=self.device)
+            for k, b in enumerate(bcs):
+                ids[torch.as_tensor(b.mask, device=self.device).bool()] = k + 1
+            self._bc_id, self._bc_key = ids, key
+        return self._bc_id, [b.value(time) for b in bcs]
+
+    def substep(self, time: float = 0.0):
+        L = _lib.lib()
+        bc_id, bc_vals = self._velocity_bcs(time)
+        if bc_id is None:
+            for bc in self.particle_bc:
+                bc(self, time)
+        n = self.n
+        ws = self._workspace()
+        for k in _STATE + _FLOAT_PARAMS:
+            self._tensor(k)
+        allow = self._tensor("allow", torch.bool)
+        imask = self._tensor("interface_mask", torch.bool)
+        do_damage = self._any_allow() and (self._step % self.damage_every == 0)
+
+        p = _lib.StepParams()
+        p.dtype, p.n_grid = self._code, self.n_grid
+        p.grid_lim, p.dx, p.inv_dx, p.dt = self.grid_lim, self.dx, self.inv_dx, self.dt
+        p.gravity[:] = self._gravity_host
+        p.grid_damp = self.grid_damp
+        if self.f_clamp is not None:
+            p.use_fclamp, p.fclamp_lo, p.fclamp_hi = 1, float(self.f_clamp[0]), float(self.f_clamp[1])
+        p.do_damage = int(do_damage)
+        p.dt_damage = self.dt * self.damage_every
+        device_cols = all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders) and \
+            len(self.colliders) <= _lib.MAX_COLLIDERS
+        if device_cols:
+            p.n_colliders = len(self.colliders)
+            for k, c in enumerate(self.colliders):
+                c.describe(time, p.colliders[k])
+        p.export_active = int(not device_cols)
+        p.n_vel_bc = len(bc_vals)
+        for k, v in enumerate(bc_vals):
+            p.vel_bc[k][:] = v
+        p.stable_sort = int(self.stable_sort)
+        p.exact_sigma_eig = int(self.exact_sigma_eig)
+
+        pin = _lib.ParticlesIn()
+        pin.n = n
+        for k in _STATE + _FLOAT_PARAMS:
+            setattr(pin, k, getattr(self, k).data_ptr())
+        pin.allow = allow.data_ptr()
+        pin.interface_mask = imask.data_ptr()
+        pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
+
+        new = {k: torch.empty_like(getattr(self, k)) for k in ("x", "v", "F", "C", "lap")}
+        new["tau_last"] = torch.empty(n, 3, 3, device=self.device, dtype=self.dtype)
+        new["J_last"] = torch.empty(n, device=self.device, dtype=self.dtype)
+        if do_damage:
+            for k in ("d", "D_hist", "phi", "phi_shear"):
+                new[k] = torch.empty(n, device=self.device, dtype=self.dtype)
+        pout = _lib.ParticlesOut()
+        for k, t in new.items():
+            setattr(pout, k, t.data_ptr())
+
+        st = _lib.stream_ptr(self.device)
+        args = (C.byref(p), C.byref(pin), C.byref(pout), ws.data_ptr(), ws.numel(), st)
+        if device_cols:
+            _lib.check(L.pmpm_substep(*args), "pmpm_substep")
+        else:
+            _lib.check(L.pmpm_substep_begin(*args), "pmpm_substep_begin")
+            self._host_colliders(time)
+            _lib.check(L.pmpm_substep_end(*args), "pmpm_substep_end")
+        for k, t in new.items():
+            setattr(self, k, t)
+        self._step += 1
+
+    def _grid_views(self):
+        gv_off, act_off = C.c_size_t(), C.c_size_t()
+        _lib.lib().pmpm_grid_views(self.n, self.n_grid, self._code, C.byref(gv_off), C.byref(act_off))
+        G = self.n_grid ** 3
+        esz = torch.finfo(self.dtype).bits // 8
+        gv = self._ws[gv_off.value:gv_off.value + G * 4 * esz].view(self.dtype).view(G, 4)
+        active = self._ws[act_off.value:act_off.value + G].view(torch.bool)
+        return gv, active
+
+    def _host_colliders(self, time):
+        """Arbitrary collider callables on the active nodes (anisompm.py:736-746)."""
+        gv, active = self._grid_views()
+        ng = self.n_grid
+        lin = torch.nonzero(active, as_tuple=False).squeeze(1)
+        pos = torch.stack([lin // (ng * ng), (lin // ng) % ng, lin % ng], 1).to(self.dtype) * self.dx
+        vs = gv[lin, :3]
+        for col in self.colliders:
+            vs = col(pos, vs, time)
+        gv[lin, :3] = vs
+
+    def run_frame(self, n_sub: int, time0: float = 0.0):
+        for s in range(n_sub):
+            self.substep(time0 + s * self.dt)
+
+    # ------------------------------------------------------------------ diagnostics
+    def last_step_stats(self):
+        """(active quadratic nodes, active cubic nodes) of the last step."""
+        q, c = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, self._code, C.byref(q),
+                                                   C.byref(c), _lib.stream_ptr(self.device)), "pmpm_last_step_stats")
+        return q.value, c.value

@@ -1,0 +1,62 @@
+"""CPU checks of the C boundary: the library loads, exports every entry point
+include/pmpm.h declares, and the ctypes structs match the C layout."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from tests.conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "pmpm.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pmpm_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_step_entry_points():
+    names = declared_functions()
+    for must in ("pmpm_substep", "pmpm_bin", "pmpm_sym_eig", "pmpm_workspace_bytes", "pmpm_last_error"):
+        assert must in names
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from peel_mpm_amd import _lib
+    L = _lib.lib()                      # also checks the struct sizes against the library's
+    for name in declared_functions():
+        assert hasattr(L, name), name
+    assert set(_lib.EXPORTS) <= set(declared_functions())
+    assert L.pmpm_version().startswith(b"pmpm")
+
+
+def test_workspace_size_and_argument_errors():
+    from peel_mpm_amd import _lib
+    L = _lib.lib()
+    small = L.pmpm_workspace_bytes(1000, 32, _lib.PMPM_F32)
+    big = L.pmpm_workspace_bytes(1 << 20, 128, _lib.PMPM_F32)
+    assert 0 < small < big
+    assert L.pmpm_workspace_bytes(1000, 32, _lib.PMPM_F64) > small
+    # argument validation fails before touching a device
+    p, pin, pout = _lib.StepParams(), _lib.ParticlesIn(), _lib.ParticlesOut()
+    p.dtype, p.n_grid = 7, 32
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 0, None)
+    assert rc == -1 and b"dtype" in L.pmpm_last_error()
+    p.dtype = _lib.PMPM_F32
+    pin.n = 10
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 0, None)
+    assert rc == -3 and b"workspace" in L.pmpm_last_error()
+    p.n_colliders = 99
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 1 << 30, None)
+    assert rc == -1 and b"collider" in L.pmpm_last_error()
+
+
+def test_product_refuses_cpu_device():
+    import torch
+    from peel_mpm_amd import AnisoMPM
+    with pytest.raises(ValueError):
+        AnisoMPM(n_grid=16, device="cpu", dtype=torch.float32)
