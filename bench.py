@@ -75,6 +75,46 @@ def time_gpu(spec, steps, warmup):
     return sim.n, wall, per_kernel, (n_q, n_c), finite
 
 
+def time_batched(n_scenes, steps, warmup):
+    """The 64-scene mix: plain AnisoMPM scene by scene vs one batched,
+    graph-captured launch set per step."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, VelocityBC, scenes
+    from peel_mpm_amd.batched import BatchedAnisoMPM
+    specs = [scenes.config3_scene(s) for s in range(n_scenes)]
+    total = sum(sp["arrays"]["x"].shape[0] for sp in specs)
+    plain = [scenes.make_solver(AnisoMPM, sp, device="cuda:0") for sp in specs]
+    for s in range(warmup):
+        for p in plain:
+            p.substep(s * 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for p in plain:
+            p.substep((warmup + s) * 1e-4)
+    torch.cuda.synchronize()
+    plain_wall = time.perf_counter() - t0
+    del plain
+    torch.cuda.empty_cache()
+    b = BatchedAnisoMPM(n_grid=64, dt=1e-4, gravity=(0.0, -9.8, 0.0), device="cuda:0", use_graph=True)
+    for sp in specs:
+        b.add_scene(scenes.tensors(sp, "cuda:0"),
+                    [VelocityBC(torch.from_numpy(m).to("cuda:0"), tuple(v)) for m, v in sp["bc"]])
+    for s in range(warmup):
+        b.substep(s * 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        b.substep((warmup + s) * 1e-4)
+    torch.cuda.synchronize()
+    batched_wall = time.perf_counter() - t0
+    finite = bool(torch.isfinite(b.x).all())
+    return {"scenes": n_scenes, "particles": total, "steps": steps,
+            "value": total * steps / batched_wall, "plain_scene_by_scene": total * steps / plain_wall,
+            "speedup_vs_plain": plain_wall / batched_wall, "ms_per_step": batched_wall / steps * 1e3,
+            "plain_ms_per_step": plain_wall / steps * 1e3, "graph": True, "finite": finite}
+
+
 def time_cpu(spec, steps):
     import torch
     from oracle import cpu_step as O
@@ -98,6 +138,9 @@ def main():
     ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2"))
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-scenes", type=int, default=64)
+    ap.add_argument("--batched-steps", type=int, default=20)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the step has no multi-GPU path in this build (--gpus {args.gpus}); "
@@ -126,6 +169,8 @@ def main():
         "active_nodes": {"quadratic": n_q, "cubic": n_c},
         "finite": finite,
     }
+    if not args.no_batched:
+        out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         cv, cores, cwall = time_cpu(spec, args.cpu_steps)
         out["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port",

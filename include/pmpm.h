@@ -72,6 +72,14 @@ typedef struct {
     int32_t exact_sigma_eig;     /* 1: Cauchy-stress eigenpairs by a second Jacobi, as the
                                     reference (anisompm.py:653); 0: closed form from the
                                     polar stretch, which shares its eigenvectors */
+    /* Batched form: n_scenes independent scenes, each on its own n_grid^3 grid,
+     * their particles concatenated; scene s owns particles
+     * [scene_offsets[s], scene_offsets[s+1]) (device int64, n_scenes+1 entries).
+     * scene_damage (device uint8 per scene, may be NULL) gates the damage update
+     * per scene when do_damage is set.  n_scenes <= 1: one scene, offsets unused. */
+    int32_t n_scenes;
+    const int64_t *scene_offsets;
+    const uint8_t *scene_damage;
 } pmpm_step_params;
 
 /* Particle state and materials at the start of the step (AnisoMPM attributes,
@@ -95,11 +103,11 @@ typedef struct {
 } pmpm_particles_out;
 
 /* ---- workspace ---------------------------------------------------------- */
-/* Bytes of workspace for up to max_n particles on an n_grid^3 grid. */
-size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype);
+/* Bytes of workspace for up to max_n particles over n_scenes grids of n_grid^3. */
+size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype);
 /* Zero the parts of the workspace whose zero state the step maintains. */
-int pmpm_workspace_init(void *ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t dtype,
-                        void *stream);
+int pmpm_workspace_init(void *ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t n_scenes,
+                        int32_t dtype, void *stream);
 
 /* ---- the step ------------------------------------------------------------ */
 /* One substep: AnisoMPM.substep (anisompm.py:593-803) minus the Python
@@ -117,21 +125,23 @@ int pmpm_substep_end(const pmpm_step_params *p, const pmpm_particles_in *in,
 /* Byte offsets into ws of the grid velocity (n_grid^3 x 4: vx,vy,vz,gd) and
  * of the dense active-node mask (n_grid^3 uint8, valid after begin with
  * export_active=1), for a host that resolves colliders itself. */
-int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t dtype, size_t *gv_offset,
-                    size_t *active_offset);
+int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
+                    size_t *gv_offset, size_t *active_offset);
 /* Active quadratic nodes (grid_m > 1e-12) and cubic nodes (sum w > 1e-12) of
  * the last step on ws; synchronises the stream. */
-int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t dtype,
-                         int64_t *n_active_q, int64_t *n_active_c, void *stream);
+int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t n_scenes,
+                         int32_t dtype, int64_t *n_active_q, int64_t *n_active_c, void *stream);
 
 /* ---- pieces exposed for function-level parity --------------------------- */
-/* Binning: bucket key = 4x4x4-cell block of floor(x*inv_dx) (clamped to the
- * grid), counting sort by key.  perm (N) lists particles bucket by bucket;
- * with stable != 0 it equals a stable argsort of key.  key (N) and
- * bucket_start ((n_grid/4 rounded up)^3 + 1) may be NULL.  The cells are
- * the reference's base cells (anisompm.py:549, :574). */
+/* Binning: counting sort of particles by cell, cells numbered bucket-major:
+ * key = bucket * 64 + (cx%4)*16 + (cy%4)*4 + (cz%4), where (cx,cy,cz) =
+ * floor(x * inv_dx), the reference's cell (anisompm.py:574), clamped to the
+ * grid, and bucket = the 4x4x4-cell block (cx/4, cy/4, cz/4) numbered
+ * (bx*nb + by)*nb + bz with nb = ceil(n_grid/4).  perm (N) lists particles
+ * cell by cell; with stable != 0 it equals a stable argsort of key.  key (N)
+ * and cell_start (64*nb^3 + 1 offsets into perm) may be NULL. */
 int pmpm_bin(const void *x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t stable,
-             int32_t *key, int32_t *perm, int32_t *bucket_start, void *ws, size_t ws_bytes,
+             int32_t *key, int32_t *perm, int32_t *cell_start, void *ws, size_t ws_bytes,
              void *stream);
 /* Stencil of each particle as the step computes it: quadratic base cell and
  * 27 node indices, cubic base cell and 64 node indices (clamp-to-edge), and

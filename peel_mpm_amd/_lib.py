@@ -36,7 +36,8 @@ class StepParams(C.Structure):
                 ("do_damage", C.c_int32), ("dt_damage", C.c_double),
                 ("n_colliders", C.c_int32), ("colliders", Collider * MAX_COLLIDERS),
                 ("n_vel_bc", C.c_int32), ("vel_bc", (C.c_double * 3) * MAX_VEL_BC),
-                ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32)]
+                ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32),
+                ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p)]
 
 
 _IN_PTRS = ("x", "v", "F", "C", "d", "lap", "D_hist", "vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0",
@@ -66,12 +67,12 @@ def lib():
     L = C.CDLL(LIB_PATH)
     vp, i32, i64, sz, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_size_t, C.c_double
     L.pmpm_workspace_bytes.restype = sz
-    L.pmpm_workspace_bytes.argtypes = [i64, i32, i32]
-    L.pmpm_workspace_init.argtypes = [vp, sz, i64, i32, i32, vp]
+    L.pmpm_workspace_bytes.argtypes = [i64, i32, i32, i32]
+    L.pmpm_workspace_init.argtypes = [vp, sz, i64, i32, i32, i32, vp]
     for f in (L.pmpm_substep, L.pmpm_substep_begin, L.pmpm_substep_end):
         f.argtypes = [C.POINTER(StepParams), C.POINTER(ParticlesIn), C.POINTER(ParticlesOut), vp, sz, vp]
-    L.pmpm_grid_views.argtypes = [i64, i32, i32, C.POINTER(sz), C.POINTER(sz)]
-    L.pmpm_last_step_stats.argtypes = [vp, i64, i32, i32, C.POINTER(i64), C.POINTER(i64), vp]
+    L.pmpm_grid_views.argtypes = [i64, i32, i32, i32, C.POINTER(sz), C.POINTER(sz)]
+    L.pmpm_last_step_stats.argtypes = [vp, i64, i32, i32, i32, C.POINTER(i64), C.POINTER(i64), vp]
     L.pmpm_bin.argtypes = [vp, i32, i64, i32, dbl, i32, vp, vp, vp, vp, sz, vp]
     L.pmpm_stencil.argtypes = [vp, i32, i64, i32, dbl, vp, vp, vp, vp, vp, vp, vp, vp]
     L.pmpm_sym_eig.argtypes = [vp, i32, i64, vp, vp, vp]

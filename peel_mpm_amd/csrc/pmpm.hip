@@ -39,6 +39,7 @@ constexpr int AR3 = AR * AR * AR;      // 343
 constexpr int WG = 256;                // threads per work-group
 constexpr int SCAN_TILE = 1024;        // buckets per scan tile (4 per thread)
 constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket kernels
+constexpr int P2G_GRID = 8192;         // one-wave work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 
 thread_local char g_err[512] = "";
@@ -94,23 +95,30 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Workspace layout (byte offsets).  The step keeps count and mark at zero
-// between steps: whoever reads them clears them.  arena holds one 6 x 343
-// slot per non-empty bucket (m, mv, sum w d, sum w), in active-list order.
+// between steps: whoever reads them clears them.  count/start are per cell
+// (bucket-major), alist/slotof/mark/blist per bucket or grid block.  arena
+// holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in
+// active-list order.
 struct Layout {
     size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, total;
-    long long NB, ntiles, G, max_slots;
-    Layout(int64_t n, int ng, int dtype) {
+    long long NB, NBs, NC, ntiles, G, Gs, max_slots;
+    // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
+    Layout(int64_t n, int ng, int dtype, int ns = 1) {
         size_t ts = dtype == PMPM_F64 ? 8 : 4;
-        G = (long long)ng * ng * ng;
-        NB = (long long)nblk(ng) * nblk(ng) * nblk(ng);
-        ntiles = (NB + SCAN_TILE - 1) / SCAN_TILE;
+        if (ns < 1) ns = 1;
+        Gs = (long long)ng * ng * ng;
+        G = Gs * ns;
+        NBs = (long long)nblk(ng) * nblk(ng) * nblk(ng);
+        NB = NBs * ns;
+        NC = NB * 64;                            // cells, bucket-major
+        ntiles = (NC + SCAN_TILE - 1) / SCAN_TILE;
         max_slots = NB < (long long)(n > 0 ? n : 1) ? NB : (long long)(n > 0 ? n : 1);
         size_t o = 0;
         arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
         gv = o;     o = al(o + (size_t)G * 4 * ts);
         active = o; o = al(o + (size_t)G);
-        count = o;  o = al(o + (size_t)NB * 4);
-        start = o;  o = al(o + (size_t)(NB + 1) * 4);
+        count = o;  o = al(o + (size_t)NC * 4);
+        start = o;  o = al(o + (size_t)(NC + 1) * 4);
         tiles = o;  o = al(o + (size_t)ntiles * 2 * 4);
         alist = o;  o = al(o + (size_t)NB * 4);
         slotof = o; o = al(o + (size_t)NB * 4);
@@ -125,36 +133,58 @@ struct Layout {
 };
 
 // ------------------------------------------------------------------ binning
+// Sort key of a particle: its cell, numbered bucket-major -- the 4x4x4-cell
+// bucket (scene-major in a batch), then the cell inside the bucket:
+//   key = (scene * NBs + bucket) * 64 + (cx & 3) * 16 + (cy & 3) * 4 + (cz & 3)
+// with the cell c = floor(x * inv_dx) of the reference (anisompm.py:574)
+// clamped to the grid's buckets.  A bucket's particles are contiguous and
+// grouped by cell, which lets P2G put one cell on each lane.
 template <typename T>
-__device__ __forceinline__ int bucket_of(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb) {
-    int b[3];
+__device__ __forceinline__ int cell_key(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb) {
+    int b[3], l[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int c = floor_idx(scaled(x[3 * p + a], inv_dx), ng);
+        const int c = floor_idx(scaled(x[3 * p + a], inv_dx), ng);
         b[a] = clampi(c >> 2, 0, nb - 1);
+        l[a] = clampi(c - BLK * b[a], 0, BLK - 1);
     }
-    return (b[0] * nb + b[1]) * nb + b[2];
+    return (((b[0] * nb + b[1]) * nb + b[2]) << 6) | (l[0] << 4) | (l[1] << 2) | l[2];
 }
 
-// Slot within the bucket: one atomic per run of equal keys in the wavefront.
+// Scene of particle p: the last s with off[s] <= p (off has ns+1 entries).
+__device__ __forceinline__ int scene_of(const long long* __restrict__ off, int ns, long long p) {
+    int lo = 0, hi = ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= p) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Slot within the cell: one atomic per run of equal keys in the wavefront.
 template <typename T>
 __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long long n, T inv_dx, int ng, int nb,
+                                                  const long long* __restrict__ scene_off, int ns,
                                                   int* __restrict__ count, int* __restrict__ key,
                                                   int* __restrict__ slot) {
     long long p = (long long)blockIdx.x * WG + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < n;
-    int k = live ? bucket_of(x, p, inv_dx, ng, nb) : -1;
+    int k = -1;
+    if (live) {
+        k = cell_key(x, p, inv_dx, ng, nb);
+        if (ns > 1) k += scene_of(scene_off, ns, p) * (nb * nb * nb * 64);
+    }
     int kprev = __shfl_up(k, 1);
     bool head = live && (lane == 0 || kprev != k);
     unsigned long long heads = __ballot(head);
     unsigned long long alive = __ballot(live);
     // run of this lane: [hl, he)
-    unsigned long long below = heads & ((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
-    int hl = 63 - __builtin_clzll(below | 1ull);            // highest head <= lane
-    unsigned long long above = heads & ~((lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1));
+    unsigned long long upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
+    int hl = 63 - __builtin_clzll((heads & upto) | 1ull);     // highest head <= lane
+    unsigned long long above = heads & ~upto;
     int he = above ? __builtin_ctzll(above) : 64;
-    int nalive_to = 64 - __builtin_clzll(alive | 1ull);      // lanes [0, nalive_to) may be live
+    int nalive_to = 64 - __builtin_clzll(alive | 1ull);      // live lanes are [0, nalive_to)
     if (he > nalive_to) he = nalive_to;
     int base = 0;
     if (head) base = atomicAdd(&count[k], he - hl);
@@ -165,18 +195,18 @@ __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long 
     }
 }
 
-// Per-tile totals of counts and of non-empty buckets.
-__global__ void __launch_bounds__(WG) k_scan_tiles(const int* __restrict__ count, long long NB, int* __restrict__ tiles) {
+// A scan tile is 1024 cells = 16 buckets; each thread holds 4 cells and 16
+// threads hold one bucket.  Per tile: particle total, non-empty buckets.
+__global__ void __launch_bounds__(WG) k_scan_tiles(const int* __restrict__ count, long long NC, int* __restrict__ tiles) {
     __shared__ int s_c[WG / 64], s_n[WG / 64];
-    long long b0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
-    int c = 0, ne = 0;
+    const long long c0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int c = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
-        if (b0 + i < NB) {
-            int v = count[b0 + i];
-            c += v;
-            ne += v > 0;
-        }
+        if (c0 + i < NC) c += count[c0 + i];
+    int bsum = c;                                  // bucket total over 16 lanes
+    for (int o = 8; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 16);
+    int ne = ((threadIdx.x & 15) == 0 && bsum > 0) ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1) {
         c += __shfl_xor(c, o);
         ne += __shfl_xor(ne, o);
@@ -191,7 +221,7 @@ __global__ void __launch_bounds__(WG) k_scan_tiles(const int* __restrict__ count
     }
 }
 
-// Exclusive block scan of (a, b) pairs held 4 per thread; returns thread's exclusive prefix.
+// Exclusive block scan of (a, b) pairs, one pair per thread.
 __device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     int ia = a, ib = b;
@@ -207,12 +237,12 @@ __device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) 
     b = pb + ib - b;
 }
 
-// Bucket starts, ordered non-empty bucket list, grid-block marks; clears count.
-__global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long long NB, int nb, const int* __restrict__ tiles,
+// Cell starts, the ordered list of non-empty buckets and their slots, marks
+// on the grid blocks their arenas reach; clears count.
+__global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long long NC, int nb, const int* __restrict__ tiles,
                                                    long long ntiles, int* __restrict__ start, int* __restrict__ alist,
                                                    int* __restrict__ slotof, int* __restrict__ mark, int* __restrict__ misc) {
     __shared__ int s_a[WG / 64], s_b[WG / 64], s_pa, s_pb;
-    // prefix of earlier tiles
     int pa = 0, pb = 0;
     for (long long t = threadIdx.x; t < blockIdx.x; t += WG) { pa += tiles[2 * t]; pb += tiles[2 * t + 1]; }
     for (int o = 32; o > 0; o >>= 1) { pa += __shfl_xor(pa, o); pb += __shfl_xor(pb, o); }
@@ -226,42 +256,50 @@ __global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long
     __syncthreads();
     const int tpa = s_pa, tpb = s_pb;
     __syncthreads();
-    long long b0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
-    int v[4], a = 0, b = 0;
+    const long long c0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+    int v[4], a = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        v[i] = b0 + i < NB ? count[b0 + i] : 0;
+        v[i] = c0 + i < NC ? count[c0 + i] : 0;
         a += v[i];
-        b += v[i] > 0;
     }
+    int bsum = a;
+    for (int o = 8; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 16);
+    const bool leader = (threadIdx.x & 15) == 0;
+    int b = (leader && bsum > 0) ? 1 : 0;
+    const int nonempty = b;
     block_scan2(a, b, s_a, s_b);
     a += tpa;
     b += tpb;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        long long bk = b0 + i;
-        if (bk < NB) {
-            start[bk] = a;
-            count[bk] = 0;
-            slotof[bk] = v[i] > 0 ? b : -1;
-            if (v[i] > 0) {
-                alist[b] = (int)bk;
-                int bx = (int)(bk / ((long long)nb * nb)), by = (int)((bk / nb) % nb), bz = (int)(bk % nb);
-                for (int dx = -1; dx <= 1; ++dx)
-                    for (int dy = -1; dy <= 1; ++dy)
-                        for (int dz = -1; dz <= 1; ++dz) {
-                            int X = bx + dx, Y = by + dy, Z = bz + dz;
-                            if (X >= 0 && X < nb && Y >= 0 && Y < nb && Z >= 0 && Z < nb)
-                                mark[(X * nb + Y) * nb + Z] = 1;
-                        }
-                ++b;
-            }
+        if (c0 + i < NC) {
+            start[c0 + i] = a;
+            count[c0 + i] = 0;
             a += v[i];
         }
     }
+    if (leader && c0 < NC) {
+        const long long bk = c0 >> 6;
+        slotof[bk] = nonempty ? b : -1;
+        if (nonempty) {
+            alist[b] = (int)bk;
+            const long long nbs = (long long)nb * nb * nb;
+            const long long sc = bk / nbs;
+            const int loc = (int)(bk - sc * nbs);
+            const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
+            for (int dx = -1; dx <= 1; ++dx)
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dz = -1; dz <= 1; ++dz) {
+                        const int X = bx + dx, Y = by + dy, Z = bz + dz;
+                        if (X >= 0 && X < nb && Y >= 0 && Y < nb && Z >= 0 && Z < nb)
+                            mark[sc * nbs + (X * nb + Y) * nb + Z] = 1;
+                    }
+        }
+    }
     if (blockIdx.x == ntiles - 1 && threadIdx.x == WG - 1) {
-        start[NB] = a;
-        misc[0] = b;
+        start[NC] = a;
+        misc[0] = b + nonempty;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) misc[1] = 0;
 }
@@ -287,50 +325,22 @@ __global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __re
     if (p < n) perm[start[key[p]] + slot[p]] = (int)p;
 }
 
-// Sort each bucket's particle ids ascending (makes perm a stable argsort of key).
-constexpr int SORT_CAP = 2048;
-__global__ void __launch_bounds__(WG) k_bin_stabilize(const int* __restrict__ start, const int* __restrict__ alist,
-                                                      const int* __restrict__ misc, int* __restrict__ perm,
-                                                      int* __restrict__ tmp) {
-    __shared__ int s[SORT_CAP];
+// Sort each cell's particle ids ascending (then perm is a stable argsort of
+// key): one lane per cell of each non-empty bucket, insertion sort in place.
+__global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ start, const int* __restrict__ alist,
+                                                      const int* __restrict__ misc, int* __restrict__ perm) {
     const int nact = misc[0];
     for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-        const int bk = alist[ai];
-        const int s0 = start[bk], cnt = start[bk + 1] - s0;
-        if (cnt <= 1) continue;
-        if (cnt <= SORT_CAP) {
-            int n2 = 1;
-            while (n2 < cnt) n2 <<= 1;
-            for (int i = threadIdx.x; i < n2; i += WG) s[i] = i < cnt ? perm[s0 + i] : 0x7fffffff;
-            __syncthreads();
-            for (int k = 2; k <= n2; k <<= 1)
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = threadIdx.x; i < n2; i += WG) {
-                        int l = i ^ j;
-                        if (l > i) {
-                            int a = s[i], b = s[l];
-                            bool up = (i & k) == 0;
-                            if ((a > b) == up) { s[i] = b; s[l] = a; }
-                        }
-                    }
-                    __syncthreads();
-                }
-            for (int i = threadIdx.x; i < cnt; i += WG) perm[s0 + i] = s[i];
-            __syncthreads();
-        } else {
-            // bucket larger than the LDS sort: each id's rank is the number of
-            // smaller ids in the bucket; place through tmp, then copy back
-            for (int i = threadIdx.x; i < cnt; i += WG) {
-                const int e = perm[s0 + i];
-                int r = 0;
-                for (int j = 0; j < cnt; ++j) r += perm[s0 + j] < e;
-                tmp[s0 + r] = e;
+        const long long c = ((long long)alist[ai] << 6) + threadIdx.x;
+        const int s0 = start[c], s1 = start[c + 1];
+        for (int i = s0 + 1; i < s1; ++i) {
+            const int e = perm[i];
+            int j = i - 1;
+            while (j >= s0 && perm[j] > e) {
+                perm[j + 1] = perm[j];
+                --j;
             }
-            __threadfence_block();
-            __syncthreads();
-            for (int i = threadIdx.x; i < cnt; i += WG) perm[s0 + i] = tmp[s0 + i];
-            __threadfence_block();
-            __syncthreads();
+            perm[j + 1] = e;
         }
     }
 }
@@ -349,8 +359,9 @@ template <typename T> struct StepArgs {
     const int *start, *alist, *slotof, *blist, *perm, *misc;
     unsigned long long* stats;
     // scalars
-    long long n;
-    int ng, nb;
+    long long n, NBs, Gs;
+    int ng, nb, nscenes;
+    const uint8_t* scene_dmg;   // per-scene damage gate, or null: all scenes run damage
     T dx, inv_dx, dt, lim, pad_lo, pad_hi, fc_lo, fc_hi, dt_dmg, damp, stress_k, c_scale, lap_scale;
     T dtg[3];
     int use_fclamp, export_active, n_col, n_bc;
@@ -392,7 +403,7 @@ template <typename T> __device__ __forceinline__ void store_m3(T* __restrict__ a
 // Returns the degraded Kirchhoff stress; F is replaced by its clamped value
 // when f_clamp is on; d becomes the post-update damage.
 template <typename T, bool DAMAGE, bool EXACT_SIGMA>
-__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J) {
+__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J, bool dmg_on) {
     M3<T> Cm = mul_at(F, F);
     T lc[3];
     M3<T> V;
@@ -439,7 +450,11 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         n0[0] = A.n_if[3 * p]; n0[1] = A.n_if[3 * p + 1]; n0[2] = A.n_if[3 * p + 2];
     }
 
-    if (DAMAGE) {
+    if (DAMAGE && !dmg_on) {
+        // this scene skips damage this step: d and D_hist pass through
+        A.od[p] = d;
+        A.oD[p] = A.D_hist[p];
+    } else if (DAMAGE) {
         const T sc = A.sigma_c[p];
         T phi, tt2 = T(0);
         if (EXACT_SIGMA) {
@@ -558,26 +573,35 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
     return tau;
 }
 
-template <typename T> __device__ __forceinline__ void lds_add(T* a, T v) { atomicAdd(a, v); }
-
-// Stress + damage + P2G (quadratic momentum, cubic damage) per bucket.
+// Stress + damage + P2G (quadratic momentum, cubic damage): one wavefront
+// per bucket, lane l <-> cell l of the bucket.  Each round every lane takes
+// the next particle of its own cell, so the 64 lanes of an LDS atomic hit
+// 64 different nodes (same-address float atomics serialise badly).
+constexpr int P2G_WG = 64;
 template <typename T, bool DAMAGE, bool EXACT_SIGMA>
-__global__ void __launch_bounds__(WG) k_p2g(StepArgs<T> A) {
+__global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
     __shared__ T s_acc[6][AR3];      // m, mvx, mvy, mvz, sum w d, sum w
     const int nact = A.misc[0];
     const int nb = A.nb, ng = A.ng;
+    const int lane = threadIdx.x;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
-        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int scene = (int)(bk / A.NBs), loc = (int)(bk - scene * A.NBs);
+        const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
-        for (int l = threadIdx.x; l < 6 * AR3; l += WG) (&s_acc[0][0])[l] = T(0);
+        const bool dmg_on = A.scene_dmg == nullptr || A.scene_dmg[scene] != 0;
+        for (int l = lane; l < 6 * AR3; l += P2G_WG) (&s_acc[0][0])[l] = T(0);
         __syncthreads();
-        const int s0 = A.start[bk], cnt = A.start[bk + 1] - s0;
-        for (int i = threadIdx.x; i < cnt; i += WG) {
-            const long long p = A.perm[s0 + i];
+        const long long cell = ((long long)bk << 6) + lane;
+        const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
+        int rounds = cnt;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        for (int r = 0; r < rounds; ++r) {
+            if (r >= cnt) continue;
+            const long long p = A.perm[c0 + r];
             M3<T> F = load_m3(A.F, p);
             T d = A.d[p], J;
-            M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J);
+            M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J, dmg_on);
             if (A.use_fclamp) store_m3(A.oF, p, F);
             store_m3(A.otau, p, tau);
             A.oJ[p] = J;
@@ -592,54 +616,63 @@ __global__ void __launch_bounds__(WG) k_p2g(StepArgs<T> A) {
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) af.m[a][b] = sk * tau.m[a][b] + mass * Cp.m[a][b];
-            T vv[3];
+            // velocity BC slot: compile-time indices only, so the by-value
+            // kernel arguments are never copied to scratch for a dynamic index
+            T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
             const int bc = A.bc_id ? A.bc_id[p] : 0;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) vv[a] = bc > 0 ? A.bcv[bc - 1][a] : A.v[3 * p + a];
+            for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
+                if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
             T mv[3] = {mass * vv[0], mass * vv[1], mass * vv[2]};
             T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx), scaled(A.x[3 * p + 2], A.inv_dx)};
-            int bq[3];
-            T fx[3], w[3][3];
+            int bq[3], bcb[3];
+            T fx[3], w[3][3], wc[4][3], w2[4][3];
             quad_weights(Xp, ng, bq, fx, w);
+            cubic_weights(Xp, ng, bcb, wc, w2);
+            // Frame of the particle: the 4 nodes [c-1, c+2] per axis of its
+            // cell c, as arena positions (not grid-clamped; positions beyond
+            // the grid fold onto the edge node in k_grid).  The quadratic
+            // stencil [b, b+2], b = c-1 or c, sits in the frame at shift s.
+            const int org[3] = {o0, o1, o2};
+            int lp[3][4], sh[3];
+            T wq[4][3], dq[4][3];
 #pragma unroll
-            for (int ox = 0; ox < 3; ++ox) {
-                const int lx = clampi(bq[0] + ox, 0, ng - 1) - o0;
+            for (int a = 0; a < 3; ++a) {
+                sh[a] = bq[a] - bcb[a];
 #pragma unroll
-                for (int oy = 0; oy < 3; ++oy) {
-                    const int ly = clampi(bq[1] + oy, 0, ng - 1) - o1;
-                    const T wxy = w[ox][0] * w[oy][1];
-#pragma unroll
-                    for (int oz = 0; oz < 3; ++oz) {
-                        const int lz = clampi(bq[2] + oz, 0, ng - 1) - o2;
-                        const int l = (lx * AR + ly) * AR + lz;
-                        const T wk = wxy * w[oz][2];
-                        const T dp[3] = {(T(ox) - fx[0]) * A.dx, (T(oy) - fx[1]) * A.dx, (T(oz) - fx[2]) * A.dx};
-                        lds_add(&s_acc[0][l], wk * mass);
-#pragma unroll
-                        for (int a = 0; a < 3; ++a)
-                            lds_add(&s_acc[1 + a][l],
-                                    wk * (mv[a] + (af.m[a][0] * dp[0] + af.m[a][1] * dp[1] + af.m[a][2] * dp[2])));
-                    }
+                for (int k = 0; k < 4; ++k) {
+                    lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
+                    const int j = k - sh[a];
+                    wq[k][a] = j == 0 ? w[0][a] : (j == 1 ? w[1][a] : (j == 2 ? w[2][a] : T(0)));
+                    dq[k][a] = (T(j) - fx[a]) * A.dx;
                 }
             }
-            // cubic scatter of the post-update damage (anisompm.py:768-776)
-            int bcb[3];
-            T wc[4][3], w2[4][3];
-            cubic_weights(Xp, ng, bcb, wc, w2);
+            // Scatter with plain LDS read-modify-write: lanes own distinct
+            // cells, so for a fixed (ox, oy, oz) the 64 lanes touch 64
+            // distinct arena entries and no atomics are needed.  Different
+            // iterations of different lanes may touch the same entry, so the
+            // accesses go through a volatile pointer: the compiler keeps
+            // their program order, and LDS executes one wave's accesses in
+            // order.
+            volatile T* acc = &s_acc[0][0];
 #pragma unroll
             for (int ox = 0; ox < 4; ++ox) {
-                const int lx = clampi(bcb[0] + ox, 0, ng - 1) - o0;
 #pragma unroll
                 for (int oy = 0; oy < 4; ++oy) {
-                    const int ly = clampi(bcb[1] + oy, 0, ng - 1) - o1;
-                    const T wxy = wc[ox][0] * wc[oy][1];
+                    const T wqxy = wq[ox][0] * wq[oy][1];
+                    const T wcxy = wc[ox][0] * wc[oy][1];
 #pragma unroll
                     for (int oz = 0; oz < 4; ++oz) {
-                        const int lz = clampi(bcb[2] + oz, 0, ng - 1) - o2;
-                        const int l = (lx * AR + ly) * AR + lz;
-                        const T wk = wxy * wc[oz][2];
-                        lds_add(&s_acc[4][l], wk * d);
-                        lds_add(&s_acc[5][l], wk);
+                        const int l = (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
+                        const T wk = wqxy * wq[oz][2];
+                        const T wck = wcxy * wc[oz][2];
+                        const T dp[3] = {dq[ox][0], dq[oy][1], dq[oz][2]};
+                        acc[l] += wk * mass;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+                            acc[(1 + c) * AR3 + l] += wk * (mv[c] + (af.m[c][0] * dp[0] + af.m[c][1] * dp[1] + af.m[c][2] * dp[2]));
+                        acc[4 * AR3 + l] += wck * d;
+                        acc[5 * AR3 + l] += wck;
                     }
                 }
             }
@@ -647,16 +680,16 @@ __global__ void __launch_bounds__(WG) k_p2g(StepArgs<T> A) {
         __syncthreads();
         // store the arena to this bucket's slot: plain coalesced stores
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
-        for (int l = threadIdx.x; l < 6 * AR3; l += WG) dst[l] = (&s_acc[0][0])[l];
+        for (int l = lane; l < 6 * AR3; l += P2G_WG) dst[l] = (&s_acc[0][0])[l];
         __syncthreads();
     }
 }
 
 // Grid velocity update and colliders per marked 4x4x4 node block
-// (anisompm.py:726-746), cubic normalisation (:777-779).  A node n (per
-// axis n = 4q + r) lies in the arena [4b-1, 4b+5] of buckets b = q-1..q for
-// r <= 1, b = q for r = 2 and b = q..q+1 for r = 3; the node's mass,
-// momentum and cubic sums are the sum over those buckets' arenas.
+// (anisompm.py:726-746), cubic normalisation (:777-779).  The nodes of block
+// (bx,by,bz) lie in the arenas [4b-1, 4b+5] of buckets b = bx-1..bx+1 per
+// axis; a node sums the arenas that cover it, in a fixed bucket order, so
+// the loop over the 27 candidate buckets is uniform across the wavefront.
 template <typename T>
 __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     const int lane = threadIdx.x & 63;
@@ -667,31 +700,49 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     unsigned long long nq = 0, nc = 0;
     for (int i = wave; i < nblocks; i += nwaves) {
         const int blk = A.blist[i];
-        const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+        const int scene = (int)(blk / A.NBs), loc = (int)(blk - scene * A.NBs);
+        const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
         if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
-        int lo[3], hi[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const int q = g[a] >> 2, r = g[a] & 3;
-            lo[a] = r <= 1 ? q - 1 : q;
-            hi[a] = r == 3 ? q + 1 : q;
-            lo[a] = lo[a] < 0 ? 0 : lo[a];
-            hi[a] = hi[a] > nb - 1 ? nb - 1 : hi[a];
-        }
         T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        for (int u = lo[0]; u <= hi[0]; ++u)
-            for (int v = lo[1]; v <= hi[1]; ++v)
-                for (int w = lo[2]; w <= hi[2]; ++w) {
-                    const int slot = A.slotof[(u * nb + v) * nb + w];
+        const int* sl = A.slotof + scene * A.NBs;
+        // arena positions of node g in bucket b: g itself, and for the edge
+        // nodes 0 and ng-1 every arena position beyond the grid (P2G keeps
+        // frame positions unclamped; the reference clamps them to the edge)
+        auto span = [&](int ga, int b, int& lo, int& hi) {
+            const int org = BLK * b - 1;
+            const int plo = ga == 0 ? org : ga;
+            const int phi = ga == ng - 1 ? org + AR - 1 : ga;
+            lo = (plo > org ? plo : org) - org;
+            hi = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
+        };
+        for (int u = bx - 1; u <= bx + 1; ++u) {
+            if (u < 0 || u >= nb) continue;
+            int xl, xh;
+            span(g[0], u, xl, xh);
+            for (int v = by - 1; v <= by + 1; ++v) {
+                if (v < 0 || v >= nb) continue;
+                int yl, yh;
+                span(g[1], v, yl, yh);
+                for (int w = bz - 1; w <= bz + 1; ++w) {
+                    if (w < 0 || w >= nb) continue;
+                    const int slot = sl[(u * nb + v) * nb + w];
                     if (slot < 0) continue;
-                    const int l = ((g[0] - (BLK * u - 1)) * AR + (g[1] - (BLK * v - 1))) * AR + (g[2] - (BLK * w - 1));
-                    const T* src = A.arena + (size_t)slot * (6 * AR3) + l;
+                    int zl, zh;
+                    span(g[2], w, zl, zh);
+                    const T* base = A.arena + (size_t)slot * (6 * AR3);
+                    for (int lx = xl; lx <= xh; ++lx)
+                        for (int ly = yl; ly <= yh; ++ly)
+                            for (int lz = zl; lz <= zh; ++lz) {
+                                const T* src = base + (lx * AR + ly) * AR + lz;
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
+                                for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
+                            }
                 }
+            }
+        }
         const int gx = g[0], gy = g[1], gz = g[2];
-        const long long node = ((long long)gx * ng + gy) * ng + gz;
+        const long long node = scene * A.Gs + ((long long)gx * ng + gy) * ng + gz;
         const T m = acc[0], mvx = acc[1], mvy = acc[2], mvz = acc[3], dsum = acc[4], wsum = acc[5];
         {
             const bool act = m > T(1e-12);
@@ -703,7 +754,9 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
                 v[2] = mvz * inv + A.dtg[2];
                 if (A.damp != T(1)) { v[0] = v[0] * A.damp; v[1] = v[1] * A.damp; v[2] = v[2] * A.damp; }
                 const T pos[3] = {T(gx) * A.dx, T(gy) * A.dx, T(gz) * A.dx};
-                for (int k = 0; k < A.n_col; ++k) {
+#pragma unroll
+                for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) {
+                    if (k >= A.n_col) break;
                     const pmpm_collider& cl = A.col[k];
                     const T p0[3] = {T(cl.p0[0]), T(cl.p0[1]), T(cl.p0[2])};
                     const T p1[3] = {T(cl.p1[0]), T(cl.p1[1]), T(cl.p1[2])};
@@ -752,7 +805,9 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
     const int nb = A.nb, ng = A.ng;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
-        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int scene = (int)(bk / A.NBs), loc = (int)(bk - scene * A.NBs);
+        const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
+        const T* gvs = A.gv + 4 * scene * A.Gs;
         const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
         for (int l = threadIdx.x; l < AR3; l += WG) {
             const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
@@ -760,12 +815,12 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
             if (gx < 0 || gy < 0 || gz < 0 || gx >= ng || gy >= ng || gz >= ng) {
                 s_g[0][l] = T(0); s_g[1][l] = T(0); s_g[2][l] = T(0); s_g[3][l] = T(0);
             } else {
-                const T* g = A.gv + 4 * (((long long)gx * ng + gy) * ng + gz);
+                const T* g = gvs + 4 * (((long long)gx * ng + gy) * ng + gz);
                 s_g[0][l] = g[0]; s_g[1][l] = g[1]; s_g[2][l] = g[2]; s_g[3][l] = g[3];
             }
         }
         __syncthreads();
-        const int s0 = A.start[bk], cnt = A.start[bk + 1] - s0;
+        const int s0 = A.start[(long long)bk << 6], cnt = A.start[((long long)bk + 1) << 6] - s0;
         for (int i = threadIdx.x; i < cnt; i += WG) {
             const long long p = A.perm[s0 + i];
             T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
@@ -846,6 +901,10 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
     }
 }
 
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
 // ------------------------------------------------------------------ test kernels
 template <typename T>
 __global__ void __launch_bounds__(WG) k_sym_eig(const T* __restrict__ A, long long n, T* __restrict__ ev, T* __restrict__ V) {
@@ -890,7 +949,8 @@ __global__ void __launch_bounds__(WG) k_stencil(const T* __restrict__ x, long lo
 inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
 
 template <typename T>
-int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const Layout& L, char* ws, hipStream_t st) {
+int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const long long* scene_off, int ns, const Layout& L,
+            char* ws, hipStream_t st) {
     const int nb = nblk(ng);
     int* count = (int*)(ws + L.count);
     int* start = (int*)(ws + L.start);
@@ -903,13 +963,15 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const La
     int* perm = (int*)(ws + L.perm);
     int* misc = (int*)(ws + L.misc);
     TimedScope ts(0, st);
-    if (n > 0) hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, count, key, slot);
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NB, tiles);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NB, nb, tiles, L.ntiles, start, alist,
+    if (n > 0)
+        hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, key,
+                           slot);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, tiles);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, nb, tiles, L.ntiles, start, alist,
                        slotof, mark, misc);
     if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm);
     if (stable && n > 0)
-        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(WG), 0, st, start, alist, misc, perm, slot);
+        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm);
     HIPCHK(hipGetLastError());
     return PMPM_OK;
 }
@@ -935,6 +997,9 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.stats = (unsigned long long*)(ws + L.misc + 16);
     A.n = in->n;
     A.ng = p->n_grid; A.nb = nblk(p->n_grid);
+    A.NBs = L.NBs; A.Gs = L.Gs;
+    A.nscenes = p->n_scenes > 1 ? p->n_scenes : 1;
+    A.scene_dmg = p->scene_damage;
     A.dx = T(p->dx); A.inv_dx = T(p->inv_dx); A.dt = T(p->dt); A.lim = T(p->grid_lim);
     A.pad_lo = T(2.0 * p->dx); A.pad_hi = T(p->grid_lim - 2.0 * p->dx);
     A.fc_lo = T(p->fclamp_lo); A.fc_hi = T(p->fclamp_hi); A.use_fclamp = p->use_fclamp;
@@ -959,7 +1024,8 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     if (in->n < 0 || in->n > 0x7fffffffLL) return fail(PMPM_EARG, "%sparticle count out of range: %lld", "", in->n);
     if (p->n_colliders < 0 || p->n_colliders > PMPM_MAX_COLLIDERS) return fail(PMPM_EARG, "%stoo many colliders: %lld", "", p->n_colliders);
     if (p->n_vel_bc < 0 || p->n_vel_bc > PMPM_MAX_VEL_BC) return fail(PMPM_EARG, "%stoo many velocity BCs: %lld", "", p->n_vel_bc);
-    Layout L(in->n, p->n_grid, p->dtype);
+    if (p->n_scenes > 1 && !p->scene_offsets) return fail(PMPM_EARG, "%sbatched step needs scene_offsets", "");
+    Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     if (in->n > 0) {
         const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, in->vol0, in->mass, in->mu, in->lam,
@@ -984,8 +1050,9 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
 template <typename T>
 int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
                int exact_sigma) {
-    Layout L(in->n, p->n_grid, p->dtype);
-    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, L, ws, st);
+    Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
+    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
+                        p->n_scenes, L, ws, st);
     if (rc) return rc;
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
     {
@@ -993,15 +1060,16 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
     }
-    HIPCHK(hipMemsetAsync(ws + L.misc + 16, 0, 16, st));
+    // a fill kernel rather than a memset, so a captured graph replays it correctly
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)(ws + L.misc + 16), 2);
     if (p->export_active) HIPCHK(hipMemsetAsync(ws + L.active, 0, (size_t)L.G, st));
     if (in->n > 0) {
         TimedScope ts(1, st);
         if (p->do_damage) {
-            if (exact_sigma) hipLaunchKernelGGL((k_p2g<T, true, true>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
-            else hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
+            if (exact_sigma) hipLaunchKernelGGL((k_p2g<T, true, true>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+            else hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
         } else {
-            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(PERSIST_WG), dim3(WG), 0, st, A);
+            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
         }
     }
     {
@@ -1014,7 +1082,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
 
 template <typename T>
 int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st) {
-    Layout L(in->n, p->n_grid, p->dtype);
+    Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
     if (in->n > 0) {
         TimedScope ts(3, st);
@@ -1040,29 +1108,31 @@ int pmpm_abi_sizes(size_t* sizes, int32_t n) {
     return k;
 }
 
-size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype) {
+size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype) {
     if (n_grid < 1 || max_n < 0) return 0;
-    return Layout(max_n, n_grid, dtype).total;
+    return Layout(max_n, n_grid, dtype, n_scenes).total;
 }
 
-int pmpm_workspace_init(void* ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t dtype, void* stream) {
+int pmpm_workspace_init(void* ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
+                        void* stream) {
     if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
-    Layout L(max_n, n_grid, dtype);
+    Layout L(max_n, n_grid, dtype, n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     HIPCHK(hipMemsetAsync(ws, 0, L.total, (hipStream_t)stream));
     return PMPM_OK;
 }
 
-int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t dtype, size_t* gv_offset, size_t* active_offset) {
-    Layout L(max_n, n_grid, dtype);
+int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype, size_t* gv_offset,
+                    size_t* active_offset) {
+    Layout L(max_n, n_grid, dtype, n_scenes);
     if (gv_offset) *gv_offset = L.gv;
     if (active_offset) *active_offset = L.active;
     return PMPM_OK;
 }
 
-int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t dtype, int64_t* n_active_q, int64_t* n_active_c,
-                         void* stream) {
-    Layout L(max_n, n_grid, dtype);
+int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype, int64_t* n_active_q,
+                         int64_t* n_active_c, void* stream) {
+    Layout L(max_n, n_grid, dtype, n_scenes);
     unsigned long long h[2] = {0, 0};
     HIPCHK(hipMemcpyAsync(h, (const char*)ws + L.misc + 16, 16, hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
@@ -1120,15 +1190,15 @@ int pmpm_timing_read(double* total_ms, int64_t* launches, int32_t n) {
 }
 
 int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t stable, int32_t* key, int32_t* perm,
-             int32_t* bucket_start, void* ws, size_t ws_bytes, void* stream) {
+             int32_t* cell_start, void* ws, size_t ws_bytes, void* stream) {
     if (!ws || (n > 0 && (!x || !perm))) return fail(PMPM_EARG, "%snull argument", "");
     if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
     if (n_grid < 4 || n_grid > 1024 || n < 0 || n > 0x7fffffffLL) return fail(PMPM_EARG, "%sbad size %lld", "", n);
     Layout L(n, n_grid, dtype);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     hipStream_t st = (hipStream_t)stream;
-    int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, L, (char*)ws, st)
-                               : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, L, (char*)ws, st);
+    int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st)
+                               : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st);
     if (rc) return rc;
     // the grid-block marks of a bare binning call are not consumed by a grid pass
     HIPCHK(hipMemsetAsync((char*)ws + L.mark, 0, (size_t)L.NB * 4, st));
@@ -1136,7 +1206,7 @@ int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv
         HIPCHK(hipMemcpyAsync(perm, (char*)ws + L.perm, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
         if (key) HIPCHK(hipMemcpyAsync(key, (char*)ws + L.key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     }
-    if (bucket_start) HIPCHK(hipMemcpyAsync(bucket_start, (char*)ws + L.start, (size_t)(L.NB + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (cell_start) HIPCHK(hipMemcpyAsync(cell_start, (char*)ws + L.start, (size_t)(L.NC + 1) * 4, hipMemcpyDeviceToDevice, st));
     return PMPM_OK;
 }
 

@@ -45,26 +45,27 @@ def stencil(x: torch.Tensor, n_grid: int, grid_lim: float = 1.0):
     return out
 
 
-def n_buckets(n_grid: int) -> int:
+def n_cells(n_grid: int) -> int:
+    """Cells of the binning key space: 64 per 4x4x4 bucket, buckets cover the grid."""
     nb = (n_grid + 3) // 4
-    return nb * nb * nb
+    return 64 * nb * nb * nb
 
 
 def bin_particles(x: torch.Tensor, n_grid: int, grid_lim: float = 1.0, stable: bool = True):
-    """Counting sort of particles by 4x4x4 cell-block.  Returns (key, perm,
-    bucket_start) int32 device tensors; with stable=True perm is the stable
-    argsort of key."""
+    """Counting sort of particles by cell, cells numbered bucket-major (4x4x4
+    blocks).  Returns (key, perm, cell_start) int32 device tensors; with
+    stable=True perm is the stable argsort of key."""
     x = x.contiguous()
     n, dev = x.shape[0], x.device
     L = _lib.lib()
     code = _lib.dtype_code(x.dtype)
-    nbytes = L.pmpm_workspace_bytes(n, n_grid, code)
+    nbytes = L.pmpm_workspace_bytes(n, n_grid, 1, code)
     ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
     st = _lib.stream_ptr(dev)
-    _lib.check(L.pmpm_workspace_init(ws.data_ptr(), nbytes, n, n_grid, code, st), "pmpm_workspace_init")
+    _lib.check(L.pmpm_workspace_init(ws.data_ptr(), nbytes, n, n_grid, 1, code, st), "pmpm_workspace_init")
     key = torch.empty(n, device=dev, dtype=torch.int32)
     perm = torch.empty(n, device=dev, dtype=torch.int32)
-    start = torch.empty(n_buckets(n_grid) + 1, device=dev, dtype=torch.int32)
+    start = torch.empty(n_cells(n_grid) + 1, device=dev, dtype=torch.int32)
     inv_dx = 1.0 / (grid_lim / n_grid)
     _lib.check(L.pmpm_bin(x.data_ptr(), code, n, n_grid, inv_dx, int(stable), key.data_ptr(), perm.data_ptr(),
                           start.data_ptr(), ws.data_ptr(), nbytes, st), "pmpm_bin")

@@ -36,7 +36,7 @@ _FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "r
 class AnisoMPM:
     def __init__(self, n_grid: int = 96, grid_lim: float = 1.0, dt: float = 1e-4, gravity=(0.0, -9.8, 0.0),
                  grid_damp: float = 1.0, f_clamp=None, damage_every: int = 1, device: str = "cuda:0",
-                 dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = False):
+                 dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = True):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the native AnisoMPM step runs on a ROCm GPU (device='cuda:N')")
@@ -158,10 +158,10 @@ class AnisoMPM:
     def _workspace(self):
         if self._ws is None or self._ws_n != self.n:
             L = _lib.lib()
-            nbytes = L.pmpm_workspace_bytes(self.n, self.n_grid, self._code)
+            nbytes = L.pmpm_workspace_bytes(self.n, self.n_grid, 1, self._code)
             self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
             self._ws_n = self.n
-            _lib.check(L.pmpm_workspace_init(self._ws.data_ptr(), nbytes, self.n, self.n_grid, self._code,
+            _lib.check(L.pmpm_workspace_init(self._ws.data_ptr(), nbytes, self.n, self.n_grid, 1, self._code,
                                              _lib.stream_ptr(self.device)), "pmpm_workspace_init")
         return self._ws
 
@@ -265,7 +265,7 @@ class AnisoMPM:
 
     def _grid_views(self):
         gv_off, act_off = C.c_size_t(), C.c_size_t()
-        _lib.lib().pmpm_grid_views(self.n, self.n_grid, self._code, C.byref(gv_off), C.byref(act_off))
+        _lib.lib().pmpm_grid_views(self.n, self.n_grid, 1, self._code, C.byref(gv_off), C.byref(act_off))
         G = self.n_grid ** 3
         esz = torch.finfo(self.dtype).bits // 8
         gv = self._ws[gv_off.value:gv_off.value + G * 4 * esz].view(self.dtype).view(G, 4)
@@ -291,6 +291,6 @@ class AnisoMPM:
     def last_step_stats(self):
         """(active quadratic nodes, active cubic nodes) of the last step."""
         q, c = C.c_int64(), C.c_int64()
-        _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, self._code, C.byref(q),
+        _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, 1, self._code, C.byref(q),
                                                    C.byref(c), _lib.stream_ptr(self.device)), "pmpm_last_step_stats")
         return q.value, c.value

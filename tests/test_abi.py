@@ -37,10 +37,11 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_workspace_size_and_argument_errors():
     from peel_mpm_amd import _lib
     L = _lib.lib()
-    small = L.pmpm_workspace_bytes(1000, 32, _lib.PMPM_F32)
-    big = L.pmpm_workspace_bytes(1 << 20, 128, _lib.PMPM_F32)
+    small = L.pmpm_workspace_bytes(1000, 32, 1, _lib.PMPM_F32)
+    big = L.pmpm_workspace_bytes(1 << 20, 128, 1, _lib.PMPM_F32)
     assert 0 < small < big
-    assert L.pmpm_workspace_bytes(1000, 32, _lib.PMPM_F64) > small
+    assert L.pmpm_workspace_bytes(1000, 32, 1, _lib.PMPM_F64) > small
+    assert L.pmpm_workspace_bytes(1000, 32, 8, _lib.PMPM_F32) > small
     # argument validation fails before touching a device
     p, pin, pout = _lib.StepParams(), _lib.ParticlesIn(), _lib.ParticlesOut()
     p.dtype, p.n_grid = 7, 32

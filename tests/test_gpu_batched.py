@@ -1,0 +1,83 @@
+"""Batched multi-scene step vs the plain step scene by scene, and HIP-graph
+replay vs eager (needs a GPU).  With the stable particle sort the step is
+deterministic: the batched launch does the same per-bucket work in the same
+order as the single-scene step, and a graph replay the same as an eager
+launch, so results must agree bit for bit."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from tests.golden_cases import rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+KEYS = ("x", "v", "F", "C", "d", "lap", "D_hist")
+
+
+def small_scenes(k=4):
+    from peel_mpm_amd import scenes
+    specs = []
+    for s in range(k):
+        sp = scenes.slab_block(n_grid=32, cells=(6 + 2 * s, 4, 8 - s), origin=(8, 8 + s, 9), ppc=8, seed=s,
+                               E=1e5 * (1 + s), sigma_c=100.0 * (1 + s))
+        specs.append(sp)
+    return specs
+
+
+def build_batched(specs, **kw):
+    from peel_mpm_amd import VelocityBC, scenes
+    from peel_mpm_amd.batched import BatchedAnisoMPM
+    b = BatchedAnisoMPM(n_grid=specs[0]["n_grid"], dt=specs[0]["dt"], gravity=specs[0]["gravity"], device=DEV, **kw)
+    for sp in specs:
+        t = scenes.tensors(sp, DEV)
+        bcs = [VelocityBC(torch.from_numpy(m).to(DEV), tuple(v)) for m, v in sp["bc"]]
+        b.add_scene(t, bcs)
+    return b
+
+
+def test_batched_matches_plain_scene_by_scene():
+    from peel_mpm_amd import AnisoMPM, scenes
+    specs = small_scenes()
+    batch = build_batched(specs)
+    plain = [scenes.make_solver(AnisoMPM, sp, device=DEV) for sp in specs]
+    # one scene without damage-allowed particles: its damage must stay untouched
+    plain[2].allow[:] = False
+    batch._parts[2][0]["allow"] = torch.zeros_like(batch._parts[2][0]["allow"])
+    for s in range(6):
+        t = s * specs[0]["dt"]
+        batch.substep(t)
+        for p in plain:
+            p.substep(t)
+    torch.cuda.synchronize()
+    for i, p in enumerate(plain):
+        for k in KEYS:
+            assert torch.equal(batch.scene(i, k), getattr(p, k)), (i, k, rel_err(batch.scene(i, k), getattr(p, k)))
+    assert float(batch.scene(2, "d").abs().max()) == 0.0
+
+
+def test_graph_replays_match_eager():
+    specs = small_scenes(3)
+    eager = build_batched(specs)
+    graph = build_batched(specs, use_graph=True)
+    for s in range(8):
+        t = s * specs[0]["dt"]
+        eager.substep(t)
+        graph.substep(t)
+        torch.cuda.synchronize()
+        # eager work between replays, every replay checked
+        for k in KEYS:
+            assert torch.equal(getattr(graph, k), getattr(eager, k)), (s, k, rel_err(getattr(graph, k), getattr(eager, k)))
+
+
+def test_batched_config3_mix_runs_and_conserves_particles():
+    from peel_mpm_amd import scenes
+    specs = [scenes.config3_scene(s) for s in range(8)]
+    b = build_batched(specs, use_graph=True)
+    n0 = b.n if hasattr(b, "n") else None
+    for s in range(5):
+        b.substep(s * 1e-4)
+    torch.cuda.synchronize()
+    assert b.n == sum(sp["arrays"]["x"].shape[0] for sp in specs)
+    assert bool(torch.isfinite(b.x).all()) and bool(torch.isfinite(b.F).all())
+    assert n0 is None or n0 == b.n

@@ -77,13 +77,16 @@ def test_stencil_cells_and_indices_bit_exact():
             np.testing.assert_allclose(s[k].cpu().numpy(), z[f"{g}_{ng}"], rtol=0, atol=2e-7)
 
 
-def expected_bucket_keys(x: torch.Tensor, ng: int):
-    """Bucket key from the oracle's cubic base cell: 4x4x4 cell blocks, clamped."""
+def expected_cell_keys(x: torch.Tensor, ng: int):
+    """Cell key from the oracle's cubic base cell (c = base_c + 1 = floor(x/dx)):
+    4x4x4-cell bucket, then the cell inside it, both clamped to the grid."""
     base_c, _, _, _ = O.cubic_weights(x.cpu(), float(ng))
     c = (base_c + 1).clamp(-4, ng + 4)
     nb = (ng + 3) // 4
     b = torch.div(c, 4, rounding_mode="floor").clamp(0, nb - 1)
-    return ((b[:, 0] * nb + b[:, 1]) * nb + b[:, 2]).numpy()
+    l = (c - 4 * b).clamp(0, 3)
+    bucket = (b[:, 0] * nb + b[:, 1]) * nb + b[:, 2]
+    return (bucket * 64 + l[:, 0] * 16 + l[:, 1] * 4 + l[:, 2]).numpy()
 
 
 @pytest.mark.parametrize("ng,n,seed", [(64, 65536, 0), (24, 1000, 1), (48, 300000, 2), (128, 1, 3), (32, 0, 4)])
@@ -97,10 +100,10 @@ def test_binning_stable_sort_bit_exact(ng, n, seed):
     xt = torch.from_numpy(x).to(DEV)
     key, perm, start = kernels.bin_particles(xt, ng, stable=True)
     torch.cuda.synchronize()
-    want = expected_bucket_keys(torch.from_numpy(x), ng)
+    want = expected_cell_keys(torch.from_numpy(x), ng)
     assert np.array_equal(key.cpu().numpy(), want)
     assert np.array_equal(perm.cpu().numpy(), np.argsort(want, kind="stable"))
-    counts = np.bincount(want, minlength=kernels.n_buckets(ng))
+    counts = np.bincount(want, minlength=kernels.n_cells(ng))
     assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
 
 
