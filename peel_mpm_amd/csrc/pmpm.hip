@@ -39,8 +39,9 @@ constexpr int AR3 = AR * AR * AR;      // 343
 constexpr int WG = 256;                // threads per work-group
 constexpr int SCAN_TILE = 1024;        // buckets per scan tile (4 per thread)
 constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket kernels
-constexpr int P2G_GRID = 8192;         // one-wave work-groups of the persistent P2G kernel
+constexpr int P2G_GRID = 2048;         // work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
+constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
 
 thread_local char g_err[512] = "";
 
@@ -100,7 +101,7 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 // holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in
 // active-list order.
 struct Layout {
-    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, total;
+    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, stats, total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -127,7 +128,8 @@ struct Layout {
         key = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         slot = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         perm = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
-        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2..3] stats (u64 x2)
+        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks
+        stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
         total = o;
     }
 };
@@ -237,25 +239,40 @@ __device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) 
     b = pb + ib - b;
 }
 
+// Exclusive scan of the per-tile (particles, non-empty buckets) totals, in
+// place; one work-group of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan_top(int* __restrict__ tiles, long long ntiles) {
+    __shared__ int s_a[16], s_b[16], s_ca, s_cb;
+    if (threadIdx.x == 0) { s_ca = 0; s_cb = 0; }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (long long t0 = 0; t0 < ntiles; t0 += 1024) {
+        const long long t = t0 + threadIdx.x;
+        const int a = t < ntiles ? tiles[2 * t] : 0, b = t < ntiles ? tiles[2 * t + 1] : 0;
+        int ia = a, ib = b;
+        for (int o = 1; o < 64; o <<= 1) {
+            int ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
+            if (lane >= o) { ia += ta; ib += tb; }
+        }
+        if (lane == 63) { s_a[w] = ia; s_b[w] = ib; }
+        __syncthreads();
+        int pa = s_ca, pb = s_cb;
+        for (int j = 0; j < w; ++j) { pa += s_a[j]; pb += s_b[j]; }
+        if (t < ntiles) { tiles[2 * t] = pa + ia - a; tiles[2 * t + 1] = pb + ib - b; }
+        __syncthreads();
+        if (threadIdx.x == 1023) { s_ca = pa + ia; s_cb = pb + ib; }
+        __syncthreads();
+    }
+}
+
 // Cell starts, the ordered list of non-empty buckets and their slots, marks
-// on the grid blocks their arenas reach; clears count.
+// on the grid blocks their arenas reach; clears count.  tiles holds the
+// exclusive prefix of earlier tiles (k_scan_top).
 __global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long long NC, int nb, const int* __restrict__ tiles,
                                                    long long ntiles, int* __restrict__ start, int* __restrict__ alist,
                                                    int* __restrict__ slotof, int* __restrict__ mark, int* __restrict__ misc) {
-    __shared__ int s_a[WG / 64], s_b[WG / 64], s_pa, s_pb;
-    int pa = 0, pb = 0;
-    for (long long t = threadIdx.x; t < blockIdx.x; t += WG) { pa += tiles[2 * t]; pb += tiles[2 * t + 1]; }
-    for (int o = 32; o > 0; o >>= 1) { pa += __shfl_xor(pa, o); pb += __shfl_xor(pb, o); }
-    if ((threadIdx.x & 63) == 0) { s_a[threadIdx.x >> 6] = pa; s_b[threadIdx.x >> 6] = pb; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int A = 0, B = 0;
-        for (int w = 0; w < WG / 64; ++w) { A += s_a[w]; B += s_b[w]; }
-        s_pa = A; s_pb = B;
-    }
-    __syncthreads();
-    const int tpa = s_pa, tpb = s_pb;
-    __syncthreads();
+    __shared__ int s_a[WG / 64], s_b[WG / 64];
+    const int tpa = tiles[2 * blockIdx.x], tpb = tiles[2 * blockIdx.x + 1];
     const long long c0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
     int v[4], a = 0;
 #pragma unroll
@@ -332,7 +349,27 @@ __global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ st
     const int nact = misc[0];
     for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
         const long long c = ((long long)alist[ai] << 6) + threadIdx.x;
-        const int s0 = start[c], s1 = start[c + 1];
+        const int s0 = start[c], s1 = start[c + 1], n = s1 - s0;
+        if (n <= 1) continue;
+        if (n <= 16) {
+            // odd-even transposition sort of up to 16 ids in registers
+            int v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = i < n ? perm[s0 + i] : 0x7fffffff;
+#pragma unroll
+            for (int round = 0; round < 16; ++round)
+#pragma unroll
+                for (int i = round & 1; i + 1 < 16; i += 2) {
+                    const int lo = v[i] < v[i + 1] ? v[i] : v[i + 1];
+                    const int hi = v[i] < v[i + 1] ? v[i + 1] : v[i];
+                    v[i] = lo;
+                    v[i + 1] = hi;
+                }
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                if (i < n) perm[s0 + i] = v[i];
+            continue;
+        }
         for (int i = s0 + 1; i < s1; ++i) {
             const int e = perm[i];
             int j = i - 1;
@@ -573,31 +610,39 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
     return tau;
 }
 
-// Stress + damage + P2G (quadratic momentum, cubic damage): one wavefront
-// per bucket, lane l <-> cell l of the bucket.  Each round every lane takes
-// the next particle of its own cell, so the 64 lanes of an LDS atomic hit
-// 64 different nodes (same-address float atomics serialise badly).
-constexpr int P2G_WG = 64;
+// Stress + damage + P2G (quadratic momentum, cubic damage) per bucket.
+// A work-group of P2G_WAVES wavefronts takes one bucket; in every wavefront
+// lane l owns cell l of the bucket, and wavefront w takes rounds
+// w, w + P2G_WAVES, ... (round r = the r-th particle of each cell).  Each
+// wavefront scatters into its own LDS arena; the arenas are summed at the
+// end.  LDS float atomics are avoided altogether (they run at a fraction of
+// the LDS rate even without address conflicts).
+constexpr int P2G_WG = 256;
 template <typename T, bool DAMAGE, bool EXACT_SIGMA>
 __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
-    __shared__ T s_acc[6][AR3];      // m, mvx, mvy, mvz, sum w d, sum w
+    constexpr int P2G_WAVES = sizeof(T) == 4 ? 4 : 2;   // fp64 arenas are twice as large
+    // per wave: node-major (m, mvx, mvy, mvz) and (sum w d, sum w), so a node's
+    // update is one 16-byte and one 8-byte LDS access (fp32)
+    __shared__ __align__(16) T s_q[P2G_WAVES][AR3][4];
+    __shared__ __align__(16) T s_c[P2G_WAVES][AR3][2];
     const int nact = A.misc[0];
     const int nb = A.nb, ng = A.ng;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
         const int scene = (int)(bk / A.NBs), loc = (int)(bk - scene * A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
         const bool dmg_on = A.scene_dmg == nullptr || A.scene_dmg[scene] != 0;
-        for (int l = lane; l < 6 * AR3; l += P2G_WG) (&s_acc[0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < P2G_WAVES * 4 * AR3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < P2G_WAVES * 2 * AR3; l += P2G_WG) (&s_c[0][0][0])[l] = T(0);
         __syncthreads();
         const long long cell = ((long long)bk << 6) + lane;
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
-        for (int r = 0; r < rounds; ++r) {
-            if (r >= cnt) continue;
+        for (int r = wv; r < rounds; r += P2G_WAVES) {
+            if (wv >= P2G_WAVES || r >= cnt) continue;
             const long long p = A.perm[c0 + r];
             M3<T> F = load_m3(A.F, p);
             T d = A.d[p], J;
@@ -650,37 +695,68 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
             // Scatter with plain LDS read-modify-write: lanes own distinct
             // cells, so for a fixed (ox, oy, oz) the 64 lanes touch 64
             // distinct arena entries and no atomics are needed.  Different
-            // iterations of different lanes may touch the same entry, so the
-            // accesses go through a volatile pointer: the compiler keeps
-            // their program order, and LDS executes one wave's accesses in
-            // order.
-            volatile T* acc = &s_acc[0][0];
+            // iterations of different lanes may touch the same entry: a
+            // compiler barrier after each node keeps the accesses in program
+            // order, and LDS executes one wavefront's accesses in order.
+            const int wsel = wv < P2G_WAVES ? wv : 0;
+            // affine momentum is linear in the node offset: af . dp =
+            // sum over axes of af[:, a] * dq[k_a][a], precomputed per axis
+            T Pa[3][3][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) Pa[a][c][k] = af.m[c][a] * dq[k][a];
 #pragma unroll
             for (int ox = 0; ox < 4; ++ox) {
+                T bx[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) bx[c] = mv[c] + Pa[0][c][ox];
 #pragma unroll
                 for (int oy = 0; oy < 4; ++oy) {
                     const T wqxy = wq[ox][0] * wq[oy][1];
                     const T wcxy = wc[ox][0] * wc[oy][1];
+                    T bxy[3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) bxy[c] = bx[c] + Pa[1][c][oy];
+                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
 #pragma unroll
                     for (int oz = 0; oz < 4; ++oz) {
-                        const int l = (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
+                        const int l = lxy + lp[2][oz];
                         const T wk = wqxy * wq[oz][2];
                         const T wck = wcxy * wc[oz][2];
-                        const T dp[3] = {dq[ox][0], dq[oy][1], dq[oz][2]};
-                        acc[l] += wk * mass;
-#pragma unroll
-                        for (int c = 0; c < 3; ++c)
-                            acc[(1 + c) * AR3 + l] += wk * (mv[c] + (af.m[c][0] * dp[0] + af.m[c][1] * dp[1] + af.m[c][2] * dp[2]));
-                        acc[4 * AR3 + l] += wck * d;
-                        acc[5 * AR3 + l] += wck;
+                        if (wk != T(0)) {
+                            T* q = s_q[wsel][l];
+                            T c0 = q[0], c1 = q[1], c2 = q[2], c3 = q[3];
+                            c0 += wk * mass;
+                            c1 += wk * (bxy[0] + Pa[2][0][oz]);
+                            c2 += wk * (bxy[1] + Pa[2][1][oz]);
+                            c3 += wk * (bxy[2] + Pa[2][2][oz]);
+                            q[0] = c0; q[1] = c1; q[2] = c2; q[3] = c3;
+                        }
+                        T* cc = s_c[wsel][l];
+                        T e0 = cc[0], e1 = cc[1];
+                        e0 += wck * d;
+                        e1 += wck;
+                        cc[0] = e0; cc[1] = e1;
+                        __asm__ __volatile__("" ::: "memory");
                     }
                 }
             }
         }
         __syncthreads();
-        // store the arena to this bucket's slot: plain coalesced stores
+        // sum the wavefront arenas in a fixed order and store this bucket's
+        // slot with plain coalesced stores
+        // slot layout: channel-major [6][343] (m, mvx, mvy, mvz, sum w d, sum w)
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
-        for (int l = lane; l < 6 * AR3; l += P2G_WG) dst[l] = (&s_acc[0][0])[l];
+        for (int l = threadIdx.x; l < 6 * AR3; l += P2G_WG) {
+            const int c = l / AR3, node = l - c * AR3;
+            T v = c < 4 ? s_q[0][node][c] : s_c[0][node][c - 4];
+#pragma unroll
+            for (int w = 1; w < P2G_WAVES; ++w) v += c < 4 ? s_q[w][node][c] : s_c[w][node][c - 4];
+            dst[l] = v;
+        }
         __syncthreads();
     }
 }
@@ -692,6 +768,12 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
 // the loop over the 27 candidate buckets is uniform across the wavefront.
 template <typename T>
 __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
+    __shared__ pmpm_collider s_col[PMPM_MAX_COLLIDERS];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) s_col[k] = A.col[k];
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
     const int nwaves = gridDim.x * (WG / 64);
@@ -706,39 +788,60 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
         if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         const int* sl = A.slotof + scene * A.NBs;
-        // arena positions of node g in bucket b: g itself, and for the edge
-        // nodes 0 and ng-1 every arena position beyond the grid (P2G keeps
-        // frame positions unclamped; the reference clamps them to the edge)
-        auto span = [&](int ga, int b, int& lo, int& hi) {
-            const int org = BLK * b - 1;
-            const int plo = ga == 0 ? org : ga;
-            const int phi = ga == ng - 1 ? org + AR - 1 : ga;
-            lo = (plo > org ? plo : org) - org;
-            hi = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
-        };
-        for (int u = bx - 1; u <= bx + 1; ++u) {
-            if (u < 0 || u >= nb) continue;
-            int xl, xh;
-            span(g[0], u, xl, xh);
-            for (int v = by - 1; v <= by + 1; ++v) {
-                if (v < 0 || v >= nb) continue;
-                int yl, yh;
-                span(g[1], v, yl, yh);
-                for (int w = bz - 1; w <= bz + 1; ++w) {
-                    if (w < 0 || w >= nb) continue;
-                    const int slot = sl[(u * nb + v) * nb + w];
-                    if (slot < 0) continue;
-                    int zl, zh;
-                    span(g[2], w, zl, zh);
-                    const T* base = A.arena + (size_t)slot * (6 * AR3);
-                    for (int lx = xl; lx <= xh; ++lx)
-                        for (int ly = yl; ly <= yh; ++ly)
-                            for (int lz = zl; lz <= zh; ++lz) {
-                                const T* src = base + (lx * AR + ly) * AR + lz;
+        // slots of the 27 candidate buckets, fetched together (uniform loads)
+        int slot[27];
 #pragma unroll
-                                for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
-                            }
+        for (int k = 0; k < 27; ++k) {
+            const int u = bx - 1 + k / 9, v = by - 1 + (k / 3) % 3, w = bz - 1 + k % 3;
+            const bool in = u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
+            slot[k] = in ? sl[(u * nb + v) * nb + w] : -1;
+        }
+        // each node's own position in every arena that holds it: all loads
+        // are independent and issued together
+#pragma unroll
+        for (int k = 0; k < 27; ++k) {
+            const int lx = g[0] - (BLK * (bx - 1 + k / 9) - 1);
+            const int ly = g[1] - (BLK * (by - 1 + (k / 3) % 3) - 1);
+            const int lz = g[2] - (BLK * (bz - 1 + k % 3) - 1);
+            if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR) {
+                const T* src = A.arena + (size_t)slot[k] * (6 * AR3) + (lx * AR + ly) * AR + lz;
+#pragma unroll
+                for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
+            }
+        }
+        // edge nodes 0 and ng-1 also collect the arena positions beyond the
+        // grid (P2G keeps frame positions unclamped; the reference clamps
+        // them onto the edge node)
+        const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
+        if (edge) {
+            for (int k = 0; k < 27; ++k) {
+                if (slot[k] < 0) continue;
+                const int b3[3] = {bx - 1 + k / 9, by - 1 + (k / 3) % 3, bz - 1 + k % 3};
+                int lo[3], hi[3];
+                bool any_extra = false, ok = true;
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const int org = BLK * b3[a] - 1;
+                    const int plo = g[a] == 0 ? org : g[a];
+                    const int phi = g[a] == ng - 1 ? org + AR - 1 : g[a];
+                    lo[a] = (plo > org ? plo : org) - org;
+                    hi[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
+                    ok = ok && lo[a] <= hi[a];
+                    any_extra = any_extra || lo[a] != hi[a] || lo[a] != g[a] - org;
                 }
+                if (!ok || !any_extra) continue;
+                const T* base = A.arena + (size_t)slot[k] * (6 * AR3);
+                for (int lx = lo[0]; lx <= hi[0]; ++lx)
+                    for (int ly = lo[1]; ly <= hi[1]; ++ly)
+                        for (int lz = lo[2]; lz <= hi[2]; ++lz) {
+                            // the node's own position was summed above
+                            if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) &&
+                                lz == g[2] - (BLK * b3[2] - 1))
+                                continue;
+                            const T* src = base + (lx * AR + ly) * AR + lz;
+#pragma unroll
+                            for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
+                        }
             }
         }
         const int gx = g[0], gy = g[1], gz = g[2];
@@ -754,10 +857,8 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
                 v[2] = mvz * inv + A.dtg[2];
                 if (A.damp != T(1)) { v[0] = v[0] * A.damp; v[1] = v[1] * A.damp; v[2] = v[2] * A.damp; }
                 const T pos[3] = {T(gx) * A.dx, T(gy) * A.dx, T(gz) * A.dx};
-#pragma unroll
-                for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) {
-                    if (k >= A.n_col) break;
-                    const pmpm_collider& cl = A.col[k];
+                for (int k = 0; k < A.n_col; ++k) {
+                    const pmpm_collider& cl = s_col[k];
                     const T p0[3] = {T(cl.p0[0]), T(cl.p0[1]), T(cl.p0[2])};
                     const T p1[3] = {T(cl.p1[0]), T(cl.p1[1]), T(cl.p1[2])};
                     const T vc[3] = {(p1[0] - p0[0]) / T(1e-4), (p1[1] - p0[1]) / T(1e-4), (p1[2] - p0[2]) / T(1e-4)};
@@ -786,14 +887,20 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
             nc += cact;
         }
     }
-    // per-wave totals of active nodes
+    // per-work-group totals of active nodes, stored (not accumulated: a
+    // same-address HBM atomic from every wave serialises the whole pass)
+    __shared__ unsigned long long s_n[2][WG / 64];
     for (int o = 32; o > 0; o >>= 1) {
         nq += __shfl_xor(nq, o);
         nc += __shfl_xor(nc, o);
     }
-    if (lane == 0 && (nq | nc)) {
-        atomicAdd(&A.stats[0], nq);
-        atomicAdd(&A.stats[1], nc);
+    if (lane == 0) { s_n[0][threadIdx.x >> 6] = nq; s_n[1][threadIdx.x >> 6] = nc; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long q = 0, c = 0;
+        for (int w = 0; w < WG / 64; ++w) { q += s_n[0][w]; c += s_n[1][w]; }
+        A.stats[2 * blockIdx.x] = q;
+        A.stats[2 * blockIdx.x + 1] = c;
     }
 }
 
@@ -967,6 +1074,7 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
         hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, key,
                            slot);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, tiles);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, L.ntiles);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, nb, tiles, L.ntiles, start, alist,
                        slotof, mark, misc);
     if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm);
@@ -994,7 +1102,7 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.start = (const int*)(ws + L.start); A.alist = (const int*)(ws + L.alist); A.perm = (const int*)(ws + L.perm);
     A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
     A.misc = (const int*)(ws + L.misc);
-    A.stats = (unsigned long long*)(ws + L.misc + 16);
+    A.stats = (unsigned long long*)(ws + L.stats);
     A.n = in->n;
     A.ng = p->n_grid; A.nb = nblk(p->n_grid);
     A.NBs = L.NBs; A.Gs = L.Gs;
@@ -1060,8 +1168,6 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
     }
-    // a fill kernel rather than a memset, so a captured graph replays it correctly
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, (unsigned long long*)(ws + L.misc + 16), 2);
     if (p->export_active) HIPCHK(hipMemsetAsync(ws + L.active, 0, (size_t)L.G, st));
     if (in->n > 0) {
         TimedScope ts(1, st);
@@ -1074,7 +1180,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     }
     {
         TimedScope ts(2, st);
-        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WAVES / (WG / 64)), dim3(WG), 0, st, A);
+        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     }
     HIPCHK(hipGetLastError());
     return PMPM_OK;
@@ -1133,9 +1239,11 @@ int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dty
 int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype, int64_t* n_active_q,
                          int64_t* n_active_c, void* stream) {
     Layout L(max_n, n_grid, dtype, n_scenes);
-    unsigned long long h[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(h, (const char*)ws + L.misc + 16, 16, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    std::vector<unsigned long long> h(2 * GRID_WG_COUNT);
+    HIPCHK(hipMemcpyAsync(h.data(), (const char*)ws + L.stats, h.size() * sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    for (int i = 1; i < GRID_WG_COUNT; ++i) { h[0] += h[2 * i]; h[1] += h[2 * i + 1]; }
     if (n_active_q) *n_active_q = (int64_t)h[0];
     if (n_active_c) *n_active_c = (int64_t)h[1];
     return PMPM_OK;
