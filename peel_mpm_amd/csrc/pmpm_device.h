@@ -137,15 +137,26 @@ template <typename T> __device__ __forceinline__ M3<T> mul_at(const M3<T>& a, co
     return r;
 }
 
+// Reciprocal and reciprocal square root of the rotation angle.  fp32 uses
+// the hardware v_rcp_f32 / v_rsq_f32 (about 1 ulp): the rotation only has to
+// be a rotation to rounding, and c, s come out of correctly rounded division
+// with 0.5 ulp error anyway.  fp64 keeps IEEE division and sqrt.
+__device__ __forceinline__ float rot_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double rot_rcp(double v) { return 1.0 / v; }
+__device__ __forceinline__ float rot_rsq(float v) { return __builtin_amdgcn_rsqf(v); }
+__device__ __forceinline__ double rot_rsq(double v) { return 1.0 / sqrt(v); }
+__device__ __forceinline__ float rot_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
+__device__ __forceinline__ double rot_sqrt(double v) { return sqrt(v); }
+
 // One Jacobi rotation zeroing a[p][q] (anisompm.py:74-91), p<q, r the third index.
 template <int p, int q, typename T> __device__ __forceinline__ void jacobi_rot(T (&a)[3][3], T (&V)[3][3]) {
     constexpr int r = 3 - p - q;
     const T app = a[p][p], aqq = a[q][q], apq = a[p][q], apr = a[p][r], aqr = a[q][r];
-    const T zeta = (aqq - app) / (T(2) * apq);
+    const T zeta = (aqq - app) * rot_rcp(T(2) * apq);
     const T sgn = zeta >= T(0) ? T(1) : T(-1);
-    T t = sgn / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+    T t = sgn * rot_rcp(fabs(zeta) + rot_sqrt(T(1) + zeta * zeta));
     t = fabs(apq) > T(1e-30) ? t : T(0);
-    const T c = T(1) / sqrt(T(1) + t * t);
+    const T c = rot_rsq(T(1) + t * t);
     const T s = c * t;
     a[p][p] = c * c * app - T(2) * s * c * apq + s * s * aqq;
     a[q][q] = s * s * app + T(2) * s * c * apq + c * c * aqq;
