@@ -132,6 +132,32 @@ int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dty
 int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t n_scenes,
                          int32_t dtype, int64_t *n_active_q, int64_t *n_active_c, void *stream);
 
+/* ---- adjoint ------------------------------------------------------------- */
+/* Adjoints of a step's outputs (read) ... */
+typedef struct {
+    const void *x, *v, *F, *C, *d, *lap, *D_hist;
+} pmpm_adjoint_in;
+/* ... and of its inputs (written), plus parameter adjoints (accumulated +=). */
+typedef struct {
+    void *x, *v, *F, *C, *d, *lap, *D_hist;
+    void *mu, *lam, *A0;
+} pmpm_adjoint_out;
+
+/* Bytes of the adjoint's workspace: 8 values per grid node, 10 per particle. */
+size_t pmpm_adjoint_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype);
+/* Backward of one substep (vector-Jacobian product of anisompm.py:593-803).
+ * No counterpart in the reference, which has no gradient code.  in = the
+ * step's inputs as passed to pmpm_substep; fwd_out = its outputs: v and C
+ * are read, d, D_hist, phi, phi_shear, tau_last, J_last are rewritten by the
+ * recompute (with the stable sort, to the same values).  p must have
+ * do_damage as in the forward call.  Supported: no f_clamp, no colliders,
+ * one scene, no interface particles (the directional split is not
+ * differentiated).  Velocity-BC particles (bc_id) get v_bar = 0. */
+int pmpm_substep_adjoint(const pmpm_step_params *p, const pmpm_particles_in *in,
+                         const pmpm_particles_out *fwd_out, const pmpm_adjoint_in *gin,
+                         const pmpm_adjoint_out *gout, void *ws, size_t ws_bytes, void *aws,
+                         size_t aws_bytes, void *stream);
+
 /* ---- pieces exposed for function-level parity --------------------------- */
 /* Binning: counting sort of particles by cell, cells numbered bucket-major:
  * key = bucket * 64 + (cx%4)*16 + (cy%4)*4 + (cz%4), where (cx,cy,cz) =

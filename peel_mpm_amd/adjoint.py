@@ -1,0 +1,239 @@
+"""Differentiable rollouts: the hand-written HIP adjoint behind autograd.
+
+``StepRunner`` advances explicit state dicts with the native step and runs
+the native backward step (``pmpm_substep_adjoint``).  ``rollout`` is a
+``torch.autograd.Function`` over ``steps`` substeps: the forward keeps a
+checkpoint every ``every`` steps; the backward recomputes each segment from
+its checkpoint, then walks it in reverse with the adjoint step.  Inputs that
+carry gradients: the initial velocity and the per-particle mu, lam and A0, so
+gradients with respect to E, nu and the fiber direction follow by ordinary
+autograd through mu = E/(2(1+nu)), lam = E nu/((1+nu)(1-2nu)) and
+A0 = I + alpha a0 a0^T (see ``material_tensors``).
+
+The reference has no gradient code; its step differentiated by torch
+autograd is NaN in every state (its Jacobi squares an overflowing zeta in
+converged rotations).  The adjoint here differentiates the invariant
+quantities instead: the polar rotation through (s_i + s_j) denominators and
+the tensile stress through Daleckii-Krein divided differences, with the
+f'(lambda) limit at equal eigenvalues, so it stays finite at F = I.
+
+Supported configurations: damage_every = 1, no f_clamp, no colliders, no
+interface particles; VelocityBC particle boundary conditions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .boundaries import VelocityBC
+from .solver import AnisoMPM
+
+STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
+_FIXED = ("vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "n_interface")
+
+
+class _AdjIn(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in STATE]
+
+
+class _AdjOut(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in STATE + ("mu", "lam", "A0")]
+
+
+def _lib_adj():
+    L = _lib.lib()
+    if not getattr(L, "_adj_ready", False):
+        L.pmpm_adjoint_workspace_bytes.restype = C.c_size_t
+        L.pmpm_adjoint_workspace_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32]
+        L.pmpm_substep_adjoint.argtypes = [C.POINTER(_lib.StepParams), C.POINTER(_lib.ParticlesIn),
+                                           C.POINTER(_lib.ParticlesOut), C.POINTER(_AdjIn), C.POINTER(_AdjOut),
+                                           C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t, C.c_void_p]
+        L._adj_ready = True
+    return L
+
+
+class StepRunner:
+    """Functional native step over explicit state dicts, using the materials,
+    configuration and VelocityBC list of an AnisoMPM ``template``."""
+
+    def __init__(self, template: AnisoMPM):
+        sim = template
+        if sim.f_clamp is not None or sim.colliders or sim.damage_every != 1:
+            raise NotImplementedError("the adjoint supports damage_every=1, no f_clamp and no colliders")
+        if bool(sim.interface_mask.any()):
+            raise NotImplementedError("the adjoint does not differentiate the directional (interface) stress")
+        if not all(isinstance(b, VelocityBC) for b in sim.particle_bc):
+            raise NotImplementedError("the adjoint takes VelocityBC particle boundary conditions only")
+        self.sim = sim
+        self.dev, self.dtype, self.n = sim.device, sim.dtype, sim.n
+        self.do_damage = bool(sim.allow.any())
+        self.allow = sim.allow.contiguous()
+        self.imask = sim.interface_mask.contiguous()
+        self.fixed = {k: getattr(sim, k).contiguous() for k in _FIXED}
+        self.params = {"mu": sim.mu.contiguous(), "lam": sim.lam.contiguous(), "A0": sim.A0.contiguous()}
+        self._ws = sim._workspace()
+        L = _lib_adj()
+        nbytes = L.pmpm_adjoint_workspace_bytes(self.n, sim.n_grid, sim._code)
+        self._aws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+
+    def set_params(self, mu, lam, A0):
+        self.params = {"mu": mu.detach().contiguous(), "lam": lam.detach().contiguous(),
+                       "A0": A0.detach().contiguous()}
+
+    def _step_params(self, k):
+        sim = self.sim
+        bc_id, bc_vals = sim._velocity_bcs(k * sim.dt)
+        p = _lib.StepParams()
+        p.dtype, p.n_grid = sim._code, sim.n_grid
+        p.grid_lim, p.dx, p.inv_dx, p.dt = sim.grid_lim, sim.dx, sim.inv_dx, sim.dt
+        p.gravity[:] = sim._gravity_host
+        p.grid_damp = sim.grid_damp
+        p.do_damage = int(self.do_damage)
+        p.dt_damage = sim.dt
+        p.n_vel_bc = len(bc_vals)
+        for j, v in enumerate(bc_vals):
+            p.vel_bc[j][:] = v
+        p.stable_sort = 1
+        return p, bc_id
+
+    def _pin(self, s, bc_id):
+        pin = _lib.ParticlesIn()
+        pin.n = self.n
+        for k in STATE:
+            setattr(pin, k, s[k].data_ptr())
+        for k, t in self.fixed.items():
+            setattr(pin, k, t.data_ptr())
+        for k, t in self.params.items():
+            setattr(pin, k, t.data_ptr())
+        pin.allow = self.allow.data_ptr()
+        pin.interface_mask = self.imask.data_ptr()
+        pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
+        return pin
+
+    def _scratch_out(self):
+        n, dev, dt = self.n, self.dev, self.dtype
+        return {"x": torch.empty(n, 3, device=dev, dtype=dt), "v": torch.empty(n, 3, device=dev, dtype=dt),
+                "F": torch.empty(n, 3, 3, device=dev, dtype=dt), "C": torch.empty(n, 3, 3, device=dev, dtype=dt),
+                "lap": torch.empty(n, device=dev, dtype=dt), "d": torch.empty(n, device=dev, dtype=dt),
+                "D_hist": torch.empty(n, device=dev, dtype=dt), "phi": torch.empty(n, device=dev, dtype=dt),
+                "phi_shear": torch.empty(n, device=dev, dtype=dt),
+                "tau_last": torch.empty(n, 3, 3, device=dev, dtype=dt), "J_last": torch.empty(n, device=dev, dtype=dt)}
+
+    def step(self, s, k):
+        """State after substep k (time k*dt) from state s."""
+        p, bc_id = self._step_params(k)
+        out = self._scratch_out()
+        pout = _lib.ParticlesOut()
+        for key, t in out.items():
+            setattr(pout, key, t.data_ptr())
+        _lib.check(_lib.lib().pmpm_substep(C.byref(p), C.byref(self._pin(s, bc_id)), C.byref(pout),
+                                           self._ws.data_ptr(), self._ws.numel(), _lib.stream_ptr(self.dev)),
+                   "pmpm_substep")
+        new = {k2: out[k2] for k2 in STATE}
+        if not self.do_damage:
+            new["d"], new["D_hist"] = s["d"], s["D_hist"]
+        return new
+
+    def step_adjoint(self, s_in, s_out, adj_out, k, grads):
+        """Adjoint of substep k: from the adjoints of s_out (adj_out) to those
+        of s_in; parameter adjoints accumulate into grads (mu, lam, A0)."""
+        p, bc_id = self._step_params(k)
+        scratch = self._scratch_out()
+        scratch["v"], scratch["C"] = s_out["v"], s_out["C"]
+        pfwd = _lib.ParticlesOut()
+        for key, t in scratch.items():
+            setattr(pfwd, key, t.data_ptr())
+        gin = _AdjIn()
+        for key in STATE:
+            setattr(gin, key, adj_out[key].data_ptr())
+        adj_in = {key: torch.empty_like(s_in[key]) for key in STATE}
+        gout = _AdjOut()
+        for key in STATE:
+            setattr(gout, key, adj_in[key].data_ptr())
+        for key in ("mu", "lam", "A0"):
+            setattr(gout, key, grads[key].data_ptr())
+        L = _lib_adj()
+        _lib.check(L.pmpm_substep_adjoint(C.byref(p), C.byref(self._pin(s_in, bc_id)), C.byref(pfwd), C.byref(gin),
+                                          C.byref(gout), self._ws.data_ptr(), self._ws.numel(), self._aws.data_ptr(),
+                                          self._aws.numel(), _lib.stream_ptr(self.dev)), "pmpm_substep_adjoint")
+        return adj_in
+
+    def zeros_like_state(self, s):
+        return {k: torch.zeros_like(s[k]) for k in STATE}
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, v0, mu, lam, A0, runner, state0, steps, every):
+        runner.set_params(mu, lam, A0)
+        s = {k: state0[k].detach().contiguous() for k in STATE}
+        s["v"] = v0.detach().contiguous()
+        ckpt = {0: s}
+        for k in range(steps):
+            s = runner.step(s, k)
+            if (k + 1) % every == 0 and k + 1 < steps:
+                ckpt[k + 1] = s
+        ctx.runner, ctx.steps, ctx.every, ctx.ckpt = runner, steps, every, ckpt
+        ctx.final = s
+        return s["x"]
+
+    @staticmethod
+    def backward(ctx, gx):
+        runner, steps, every = ctx.runner, ctx.steps, ctx.every
+        final = ctx.final
+        adj = runner.zeros_like_state(final)
+        adj["x"] = gx.contiguous()
+        grads = {k: torch.zeros_like(v) for k, v in runner.params.items()}
+        starts = list(range(0, steps, every))
+        for s0 in reversed(starts):
+            s1 = min(s0 + every, steps)
+            states = [ctx.ckpt[s0]]
+            for k in range(s0, s1):
+                states.append(runner.step(states[-1], k))
+            for k in reversed(range(s0, s1)):
+                adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads)
+        ctx.adj0 = adj
+        return adj["v"], grads["mu"], grads["lam"], grads["A0"], None, None, None, None
+
+
+def rollout(sim: AnisoMPM, v0, mu, lam, A0, steps: int, every: int = 10):
+    """Final positions after ``steps`` substeps from ``sim``'s current state,
+    differentiable in v0, mu, lam, A0 (checkpoint every ``every`` steps)."""
+    runner = StepRunner(sim)
+    state0 = {k: getattr(sim, k) for k in STATE}
+    return _Rollout.apply(v0, mu, lam, A0, runner, state0, steps, every)
+
+
+def material_tensors(E, nu, a0, alpha):
+    """mu, lam and A0 from Young's modulus, Poisson ratio and fibers as
+    AnisoMPM.add_object builds them (anisompm.py:371-412), as torch ops."""
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    eye = torch.eye(3, device=a0.device, dtype=a0.dtype)
+    A0 = eye + alpha.view(-1, 1, 1) * (a0.unsqueeze(2) * a0.unsqueeze(1))
+    return mu, lam, A0
+
+
+def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10):
+    """Adjoint of the whole initial state and of mu, lam, A0 for the vector
+    x_bar on the positions after ``steps`` substeps (no autograd graph)."""
+    runner = StepRunner(sim)
+    s = {k: getattr(sim, k).contiguous() for k in STATE}
+    ckpt = {0: s}
+    for k in range(steps):
+        s = runner.step(s, k)
+        if (k + 1) % every == 0 and k + 1 < steps:
+            ckpt[k + 1] = s
+    adj = runner.zeros_like_state(s)
+    adj["x"] = x_bar.contiguous()
+    grads = {k: torch.zeros_like(v) for k, v in runner.params.items()}
+    for s0 in reversed(range(0, steps, every)):
+        s1 = min(s0 + every, steps)
+        states = [ckpt[s0]]
+        for k in range(s0, s1):
+            states.append(runner.step(states[-1], k))
+        for k in reversed(range(s0, s1)):
+            adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads)
+    return s, adj, grads

@@ -392,6 +392,7 @@ template <typename T> struct StepArgs {
     T *ox, *ov, *oF, *oC, *olap, *od, *oD, *ophi, *ophs, *otau, *oJ;
     // workspace
     T *arena, *gv;
+    T* gmw;                     // optional: per node (m, sum w) of the grid pass
     uint8_t* active;
     const int *start, *alist, *slotof, *blist, *perm, *misc;
     unsigned long long* stats;
@@ -883,6 +884,7 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
             g[0] = v[0]; g[1] = v[1]; g[2] = v[2];
             g[3] = cact ? dsum / wsum : T(0);
             if (A.export_active) A.active[node] = act ? 1 : 0;
+            if (A.gmw) { A.gmw[2 * node] = m; A.gmw[2 * node + 1] = wsum; }
             nq += act;
             nc += cact;
         }
@@ -1011,6 +1013,8 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
 __global__ void k_zero_u64(unsigned long long* p, int n) {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
+
+#include "pmpm_adjoint.h"
 
 // ------------------------------------------------------------------ test kernels
 template <typename T>
@@ -1198,6 +1202,43 @@ int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_
     return PMPM_OK;
 }
 
+template <typename T>
+int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* fwd,
+                 const pmpm_adjoint_in* gin, const pmpm_adjoint_out* gout, char* ws, char* aws, hipStream_t st) {
+    Layout L(in->n, p->n_grid, p->dtype, 1);
+    // recompute the step's binning, P2G and grid pass, exporting (m, sum w)
+    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, 1, nullptr, 1, L, ws, st);
+    if (rc) return rc;
+    StepArgs<T> A = make_args<T>(p, in, fwd, L, ws);
+    A.gmw = (T*)aws;
+    hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
+                       (int*)(ws + L.blist), (int*)(ws + L.misc));
+    if (in->n > 0) {
+        if (p->do_damage) hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+    }
+    hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
+    AdjArgs<T> B;
+    B.gx = (const T*)gin->x; B.gv = (const T*)gin->v; B.gF = (const T*)gin->F; B.gC = (const T*)gin->C;
+    B.gd = (const T*)gin->d; B.glap = (const T*)gin->lap; B.gD = (const T*)gin->D_hist;
+    B.nv = (const T*)fwd->v; B.nC = (const T*)fwd->C;
+    B.ox = (T*)gout->x; B.ov = (T*)gout->v; B.oF = (T*)gout->F; B.oC = (T*)gout->C; B.od = (T*)gout->d;
+    B.olap = (T*)gout->lap; B.oD = (T*)gout->D_hist; B.omu = (T*)gout->mu; B.olam = (T*)gout->lam;
+    B.oA0 = (T*)gout->A0;
+    B.gmw = (const T*)aws;
+    B.gadj = (T*)aws + 2 * L.G;
+    B.pbar = (T*)aws + 8 * L.G;
+    B.dmg_on = p->do_damage;
+    if (in->n > 0) {
+        hipLaunchKernelGGL(k_adj_scatter<T>, dim3(P2G_GRID), dim3(P2G_WG), 0, st, A, B);
+        hipLaunchKernelGGL(k_adj_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A, B);
+        hipLaunchKernelGGL(k_adj_gather<T>, dim3(nblocks(in->n)), dim3(WG), 0, st, A, B);
+        hipLaunchKernelGGL(k_adj_const<T>, dim3(nblocks(in->n)), dim3(WG), 0, st, A, B);
+    }
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -1295,6 +1336,36 @@ int pmpm_timing_read(double* total_ms, int64_t* launches, int32_t n) {
         if (launches) launches[g] = (int64_t)(v.size() / 2);
     }
     return PMPM_OK;
+}
+
+size_t pmpm_adjoint_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype) {
+    if (n_grid < 1 || max_n < 0) return 0;
+    const size_t ts = dtype == PMPM_F64 ? 8 : 4;
+    return ((size_t)n_grid * n_grid * n_grid * 8 + (size_t)max_n * 10) * ts;
+}
+
+int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* fwd_out,
+                         const pmpm_adjoint_in* gin, const pmpm_adjoint_out* gout, void* ws, size_t ws_bytes, void* aws,
+                         size_t aws_bytes, void* stream) {
+    int64_t n;
+    int rc = check_step(p, in, fwd_out, ws_bytes, n);
+    if (rc) return rc;
+    if (!ws || !aws || !gin || !gout) return fail(PMPM_EARG, "%snull argument", "");
+    if (aws_bytes < pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype))
+        return fail(PMPM_ESIZE, "%sadjoint workspace too small, need %lld bytes", "",
+                    (long long)pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype));
+    if (p->use_fclamp || p->n_colliders > 0 || p->n_scenes > 1)
+        return fail(PMPM_EARG, "%sthe adjoint supports no f_clamp, no colliders and one scene", "");
+    if (in->n > 0) {
+        const void* need[] = {gin->x, gin->v, gin->F, gin->C, gin->d, gin->lap, gin->D_hist, gout->x, gout->v, gout->F,
+                              gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, fwd_out->d,
+                              fwd_out->D_hist, fwd_out->phi, fwd_out->phi_shear};
+        for (const void* q : need)
+            if (!q) return fail(PMPM_EARG, "%smissing adjoint array", "");
+    }
+    return p->dtype == PMPM_F64
+               ? step_adjoint<double>(p, in, fwd_out, gin, gout, (char*)ws, (char*)aws, (hipStream_t)stream)
+               : step_adjoint<float>(p, in, fwd_out, gin, gout, (char*)ws, (char*)aws, (hipStream_t)stream);
 }
 
 int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv_dx, int32_t stable, int32_t* key, int32_t* perm,

@@ -16,7 +16,9 @@ in the tests):
 * one ``substep`` (anisompm.py:593-803) on small scenes with every branch
   toggled (damage on/off, ``damage_every``, ``f_clamp``, directional stress,
   grid damping, colliders, particle BC), in fp32 and fp64;
-* a short multi-step trajectory of a small peel slab.
+* a short multi-step trajectory of a small peel slab;
+* autograd through a 3-step fp64 rollout (the reference has no gradient
+  code of its own; this pins the hand-written adjoint).
 
 Run:  python tests/golden/make_goldens.py
 """
@@ -279,6 +281,84 @@ def gold_traj(ref):
         np.savez_compressed(os.path.join(HERE, f"traj_slab{tag}.npz"), **rec)
 
 
+def grad_safe_sym_eig(A, sweeps=5):
+    """The reference's cyclic Jacobi (anisompm.py:48-95) with gradient-safe
+    masked branches.  The reference's own autograd is NaN in every state: once
+    an off-diagonal converges, zeta*zeta overflows (or apq == 0 gives 0/0) and
+    the backward of the masked branch forms 0 * inf.  Here apq is replaced by 1
+    where masked and |zeta| is capped at 1e150 before squaring; the forward
+    values change by at most ~1e-300 (t ~ 1/(2 zeta) there)."""
+    A = torch.nan_to_num(0.5 * (A + A.transpose(1, 2)))
+    N = A.shape[0]
+    Am = A.clone()
+    V = torch.eye(3, dtype=A.dtype).expand(N, 3, 3).contiguous()
+    for _ in range(sweeps):
+        for (p, q) in ((0, 1), (0, 2), (1, 2)):
+            r = 3 - p - q
+            app = Am[:, p, p].clone(); aqq = Am[:, q, q].clone(); apq = Am[:, p, q].clone()
+            apr = Am[:, p, r].clone(); aqr = Am[:, q, r].clone()
+            live = apq.abs() > 1e-30
+            zeta = (aqq - app) / (2.0 * torch.where(live, apq, torch.ones_like(apq)))
+            cap = torch.full_like(zeta, 1e150)
+            zeta = torch.where(zeta.abs() < 1e150, zeta, torch.where(zeta >= 0, cap, -cap))
+            sgn = torch.where(zeta >= 0, torch.ones_like(zeta), -torch.ones_like(zeta))
+            t = sgn / (zeta.abs() + torch.sqrt(1.0 + zeta * zeta))
+            t = torch.where(live, t, torch.zeros_like(t))
+            c = 1.0 / torch.sqrt(1.0 + t * t); s = c * t
+            Am[:, p, p] = c * c * app - 2.0 * s * c * apq + s * s * aqq
+            Am[:, q, q] = s * s * app + 2.0 * s * c * apq + c * c * aqq
+            Am[:, p, q] = 0.0; Am[:, q, p] = 0.0
+            npr = c * apr - s * aqr; nqr = s * apr + c * aqr
+            Am[:, p, r] = npr; Am[:, r, p] = npr
+            Am[:, q, r] = nqr; Am[:, r, q] = nqr
+            vp = V[:, :, p].clone(); vq = V[:, :, q].clone()
+            V[:, :, p] = c.unsqueeze(1) * vp - s.unsqueeze(1) * vq
+            V[:, :, q] = s.unsqueeze(1) * vp + c.unsqueeze(1) * vq
+    evals = torch.diagonal(Am, dim1=1, dim2=2)
+    evals, idx = torch.sort(evals, dim=1)
+    V = torch.gather(V, 2, idx.unsqueeze(1).expand(-1, 3, -1))
+    return evals, V
+
+
+def gold_adjoint(ref):
+    """Autograd through the reference step (with grad_safe_sym_eig in place of
+    its sym_eig): 3 fp64 substeps of a random, non-degenerate damaging scene
+    (no f_clamp, colliders or interface), loss L = sum |x_3 - x_target|^2;
+    gradients w.r.t. the initial state and the per-particle mu, lam, A0."""
+    saved = ref.sym_eig
+    ref.sym_eig = grad_safe_sym_eig
+    try:
+        _gold_adjoint(ref)
+    finally:
+        ref.sym_eig = saved
+
+
+def _gold_adjoint(ref):
+    sim, rng = random_scene(ref, 8, torch.float64, ng=24, n_side=(8, 6, 8), gravity=(0.0, -9.8, 0.0))
+    n = sim.n
+    sim.sigma_c = sim.sigma_c * torch.tensor(rng.uniform(0.5, 40.0, n))   # some particles above the threshold
+    leaves = {}
+    for k in ("x", "v", "F", "C", "d", "lap", "D_hist", "mu", "lam", "A0"):
+        t = getattr(sim, k).detach().clone().requires_grad_(True)
+        setattr(sim, k, t)
+        leaves[k] = t
+    rec = {"in_" + k: np_(v) for k, v in leaves.items()}
+    for k in ("vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "allow", "interface_mask", "n_interface"):
+        rec["in_" + k] = np_(getattr(sim, k))
+    target = leaves["x"].detach() + torch.tensor(rng.normal(0, 1e-3, (n, 3)))
+    for s in range(3):
+        sim.substep(s * sim.dt)
+    loss = ((sim.x - target) ** 2).sum()
+    loss.backward()
+    rec["target"] = np_(target)
+    rec["loss"] = np.float64(loss.item())
+    rec["out_x"] = np_(sim.x)
+    for k, t in leaves.items():
+        rec["grad_" + k] = np_(t.grad)
+    rec["cfg"] = np.array([sim.n_grid, sim.grid_lim, sim.dt, 3], np.float64)
+    np.savez_compressed(os.path.join(HERE, "adjoint_rollout64.npz"), **rec)
+
+
 def main():
     torch.set_num_threads(1)
     ref = load_reference()
@@ -286,6 +366,7 @@ def main():
     gold_weights(ref)
     gold_steps(ref)
     gold_traj(ref)
+    gold_adjoint(ref)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

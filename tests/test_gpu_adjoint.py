@@ -1,0 +1,175 @@
+"""Hand-written adjoint of the step (needs a GPU), in fp64:
+
+* against autograd through the reference step (golden vectors: 3 substeps of
+  a damaging scene; the reference's sym_eig swapped for a gradient-safe twin,
+  since the reference's own autograd is NaN in every state) -- all state and
+  parameter gradients within 1e-6 norm-wise relative error;
+* against fp64 central finite differences of the loss along random
+  directions: a 20-step elastic pulled-slab rollout (h = 1e-4) and 3
+  damaging substeps (h = 1e-6..1e-4, below the damage update's
+  relu/max/clamp kinks): 1e-4;
+* the autograd.Function with checkpointing gives the same gradients for any
+  checkpoint spacing.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from tests.golden_cases import load_npz, rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
+
+
+def sim_from_golden():
+    from peel_mpm_amd import AnisoMPM
+    z = load_npz("adjoint_rollout64.npz")
+    ng, lim, dt, steps = z["cfg"].tolist()
+    sim = AnisoMPM(n_grid=int(ng), grid_lim=lim, dt=dt, device=DEV, dtype=torch.float64)
+    for k in STATE + ("mu", "lam", "A0", "vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "allow",
+                      "interface_mask", "n_interface"):
+        setattr(sim, k, torch.from_numpy(z["in_" + k].copy()).to(DEV))
+    sim.n = sim.x.shape[0]
+    return sim, z, int(steps)
+
+
+def test_adjoint_matches_reference_autograd():
+    from peel_mpm_amd.adjoint import state_vjp
+    sim, z, steps = sim_from_golden()
+    target = torch.from_numpy(z["target"]).to(DEV)
+    # forward once to get x_T, then the VJP of L = |x_T - target|^2
+    from peel_mpm_amd.adjoint import StepRunner
+    runner = StepRunner(sim)
+    s = {k: getattr(sim, k) for k in STATE}
+    for k in range(steps):
+        s = runner.step(s, k)
+    loss = float(((s["x"] - target) ** 2).sum())
+    assert abs(loss - float(z["loss"])) <= 1e-9 * abs(float(z["loss"]))
+    _, adj, grads = state_vjp(sim, steps, 2.0 * (s["x"] - target))
+    torch.cuda.synchronize()
+    report = {}
+    for k in STATE:
+        report[k] = rel_err(adj[k], torch.from_numpy(z["grad_" + k]))
+    for k in ("mu", "lam", "A0"):
+        report[k] = rel_err(grads[k], torch.from_numpy(z["grad_" + k]))
+    print({k: f"{v:.2e}" for k, v in report.items()})
+    for k, e in report.items():
+        assert e <= 1e-6, (k, e, report)
+
+
+def fd_scene(steps=20):
+    from peel_mpm_amd import AnisoMPM, scenes
+    spec = scenes.slab_block(n_grid=32, cells=(8, 4, 8), origin=(12, 10, 12), ppc=4, seed=3, steps=steps,
+                             dtype=torch.float64)
+    return scenes.make_solver(AnisoMPM, spec, device=DEV), spec
+
+
+def loss_of(sim, v0, mu, lam, A0, steps, target):
+    from peel_mpm_amd.adjoint import StepRunner
+    runner = StepRunner(sim)
+    runner.set_params(mu, lam, A0)
+    s = {k: getattr(sim, k) for k in STATE}
+    s["v"] = v0
+    for k in range(steps):
+        s = runner.step(s, k)
+    return float(((s["x"] - target) ** 2).sum())
+
+
+@pytest.mark.parametrize("which", ["v0", "E", "nu"])
+def test_adjoint_vs_central_finite_differences_elastic(which):
+    """20-step pulled slab with damage off (a smooth loss): <grad, delta> vs
+    central differences at h = 1e-4 along a random direction."""
+    from peel_mpm_amd.adjoint import material_tensors, rollout
+    steps = 20
+    sim, spec = fd_scene(steps)
+    sim.allow[:] = False
+    n = sim.n
+    g = torch.Generator(device="cpu").manual_seed(7)
+    E = torch.full((n,), 1e5, dtype=torch.float64, device=DEV)
+    nu = torch.full((n,), 0.3, dtype=torch.float64, device=DEV)
+    v0 = (0.05 * torch.randn(n, 3, generator=g, dtype=torch.float64)).to(DEV)
+    target = (sim.x + 0.01 * torch.randn(n, 3, generator=g, dtype=torch.float64).to(DEV)).detach()
+    leaves = {"v0": v0.clone().requires_grad_(True), "E": E.clone().requires_grad_(True),
+              "nu": nu.clone().requires_grad_(True)}
+    mu, lam, A0 = material_tensors(leaves["E"], leaves["nu"], sim.a0, sim.alpha)
+    xT = rollout(sim, leaves["v0"], mu, lam, A0, steps, every=7)
+    ((xT - target) ** 2).sum().backward()
+    grad = leaves[which].grad
+    assert bool(torch.isfinite(grad).all())
+    scale = {"v0": 0.05, "E": 1e5, "nu": 0.3}[which]
+    delta = torch.randn(grad.shape, generator=g, dtype=torch.float64).to(DEV) * scale
+    h = 1e-4
+
+    def L_at(eps):
+        vals = {k: t.detach().clone() for k, t in leaves.items()}
+        vals[which] = vals[which] + eps * delta
+        m, l, A = material_tensors(vals["E"], vals["nu"], sim.a0, sim.alpha)
+        return loss_of(sim, vals["v0"], m, l, A, steps, target)
+
+    fd = (L_at(h) - L_at(-h)) / (2 * h)
+    an = float((grad * delta).sum())
+    print(which, "adjoint", an, "fd", fd)
+    assert abs(an - fd) <= 1e-4 * abs(fd), (which, an, fd)
+
+
+@pytest.mark.parametrize("which", ["mu", "lam", "A0", "v", "F", "d"])
+def test_adjoint_vs_central_finite_differences_damage(which):
+    """3 damaging substeps from the golden scene's random state: directional
+    derivative of the loss vs central differences (h = 1e-6 to 1e-4
+    relative: small enough not to cross the damage update's relu/max/clamp
+    kinks, large enough to stay above round-off)."""
+    from peel_mpm_amd.adjoint import StepRunner, state_vjp
+    sim, z, steps = sim_from_golden()
+    target = torch.from_numpy(z["target"]).to(DEV)
+    runner = StepRunner(sim)
+    s = {k: getattr(sim, k) for k in STATE}
+    for k in range(steps):
+        s = runner.step(s, k)
+    _, adj, grads = state_vjp(sim, steps, 2.0 * (s["x"] - target))
+    grad = grads[which] if which in grads else adj[which]
+    g = torch.Generator(device="cpu").manual_seed(11)
+    base = getattr(sim, which)
+    delta = torch.randn(base.shape, generator=g, dtype=torch.float64).to(DEV) * float(base.abs().mean())
+    if which == "d":
+        # half the particles sit on the clamp boundary d = 0, where torch's
+        # (inclusive) clamp subgradient and a central difference differ
+        delta = delta * (base > 0.01).to(delta.dtype)
+    # mu, lam move the loss little over 3 steps: a larger step keeps the
+    # difference above fp64 round-off of the loss
+    h = {"mu": 1e-4, "lam": 1e-4, "d": 1e-5}.get(which, 1e-6)
+
+    def L_at(eps):
+        old = getattr(sim, which)
+        setattr(sim, which, old + eps * delta)
+        r = StepRunner(sim)
+        st = {k: getattr(sim, k) for k in STATE}
+        for k in range(steps):
+            st = r.step(st, k)
+        setattr(sim, which, old)
+        return float(((st["x"] - target) ** 2).sum())
+
+    fd = (L_at(h) - L_at(-h)) / (2 * h)
+    an = float((grad * delta).sum())
+    print(which, "adjoint", an, "fd", fd)
+    assert abs(an - fd) <= 1e-4 * abs(fd), (which, an, fd)
+
+
+def test_checkpoint_spacing_does_not_change_gradients():
+    from peel_mpm_amd.adjoint import material_tensors, rollout
+    steps = 12
+    sim, spec = fd_scene(steps)
+    n = sim.n
+    target = (sim.x + 0.01).detach()
+    res = []
+    for every in (1, 5, 12):
+        E = torch.full((n,), 1e5, dtype=torch.float64, device=DEV, requires_grad=True)
+        nu = torch.full((n,), 0.3, dtype=torch.float64, device=DEV)
+        mu, lam, A0 = material_tensors(E, nu, sim.a0, sim.alpha)
+        xT = rollout(sim, sim.v.clone(), mu, lam, A0, steps, every=every)
+        ((xT - target) ** 2).sum().backward()
+        res.append(E.grad.clone())
+    for r in res[1:]:
+        assert torch.equal(r, res[0])
