@@ -115,6 +115,41 @@ def time_batched(n_scenes, steps, warmup):
             "plain_ms_per_step": plain_wall / steps * 1e3, "graph": True, "finite": finite}
 
 
+def time_adjoint(steps, every):
+    """The differentiable form: config4 (32,768 particles), a forward rollout
+    of ``steps`` substeps vs its backward pass (segment recompute from the
+    checkpoints every ``every`` steps + the adjoint steps)."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, scenes
+    from peel_mpm_amd.adjoint import material_tensors, rollout
+    spec = scenes.config4(dtype=torch.float32)
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
+    n = sim.n
+    E = torch.full((n,), 1e5, device="cuda:0", requires_grad=True)
+    nu = torch.full((n,), 0.3, device="cuda:0", requires_grad=True)
+    a0 = sim.a0.clone().requires_grad_(True)
+    v0 = sim.v.clone().requires_grad_(True)
+    target = sim.x.detach() + 0.01
+    res = {}
+    for rep in range(2):                   # first pass warms up allocations and code objects
+        mu, lam, A0 = material_tensors(E, nu, a0, sim.alpha)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xT = rollout(sim, v0, mu, lam, A0, steps, every=every)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ((xT - target) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        res = {"fwd": t1 - t0, "bwd": t2 - t1}
+    finite = all(bool(torch.isfinite(t.grad).all()) for t in (E, nu, a0, v0))
+    return {"workload": f"config4: {n} particles, 64^3 grid, {steps} steps, checkpoint every {every}",
+            "forward_ms_per_step": res["fwd"] / steps * 1e3, "backward_ms_per_step": res["bwd"] / steps * 1e3,
+            "adjoint_forward_ratio": res["bwd"] / res["fwd"],
+            "forward_plus_adjoint_particle_steps_per_s": n * steps / (res["fwd"] + res["bwd"]),
+            "grads_finite": finite}
+
+
 def time_cpu(spec, steps):
     import torch
     from oracle import cpu_step as O
@@ -141,6 +176,8 @@ def main():
     ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=20)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the step has no multi-GPU path in this build (--gpus {args.gpus}); "
@@ -171,6 +208,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
+    if not args.no_adjoint:
+        out["adjoint"] = time_adjoint(args.adjoint_steps, 10)
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         cv, cores, cwall = time_cpu(spec, args.cpu_steps)
         out["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port",

@@ -140,7 +140,7 @@ typedef struct {
 /* ... and of its inputs (written), plus parameter adjoints (accumulated +=). */
 typedef struct {
     void *x, *v, *F, *C, *d, *lap, *D_hist;
-    void *mu, *lam, *A0;
+    void *mu, *lam, *A0, *sigma_c;
 } pmpm_adjoint_out;
 
 /* Bytes of the adjoint's workspace: 8 values per grid node, 10 per particle. */

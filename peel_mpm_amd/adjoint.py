@@ -5,7 +5,8 @@ the native backward step (``pmpm_substep_adjoint``).  ``rollout`` is a
 ``torch.autograd.Function`` over ``steps`` substeps: the forward keeps a
 checkpoint every ``every`` steps; the backward recomputes each segment from
 its checkpoint, then walks it in reverse with the adjoint step.  Inputs that
-carry gradients: the initial velocity and the per-particle mu, lam and A0, so
+carry gradients: the initial velocity and the per-particle mu, lam, A0 and
+damage threshold sigma_c, so
 gradients with respect to E, nu and the fiber direction follow by ordinary
 autograd through mu = E/(2(1+nu)), lam = E nu/((1+nu)(1-2nu)) and
 A0 = I + alpha a0 a0^T (see ``material_tensors``).
@@ -31,15 +32,18 @@ from .boundaries import VelocityBC
 from .solver import AnisoMPM
 
 STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
-_FIXED = ("vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "n_interface")
+_FIXED = ("vol0", "mass", "eta", "zeta", "l0", "residual", "n_interface")
 
 
 class _AdjIn(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in STATE]
 
 
+PARAMS = ("mu", "lam", "A0", "sigma_c")
+
+
 class _AdjOut(C.Structure):
-    _fields_ = [(k, C.c_void_p) for k in STATE + ("mu", "lam", "A0")]
+    _fields_ = [(k, C.c_void_p) for k in STATE + PARAMS]
 
 
 def _lib_adj():
@@ -72,15 +76,16 @@ class StepRunner:
         self.allow = sim.allow.contiguous()
         self.imask = sim.interface_mask.contiguous()
         self.fixed = {k: getattr(sim, k).contiguous() for k in _FIXED}
-        self.params = {"mu": sim.mu.contiguous(), "lam": sim.lam.contiguous(), "A0": sim.A0.contiguous()}
+        self.params = {k: getattr(sim, k).contiguous() for k in PARAMS}
         self._ws = sim._workspace()
         L = _lib_adj()
         nbytes = L.pmpm_adjoint_workspace_bytes(self.n, sim.n_grid, sim._code)
         self._aws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
 
-    def set_params(self, mu, lam, A0):
+    def set_params(self, mu, lam, A0, sigma_c=None):
         self.params = {"mu": mu.detach().contiguous(), "lam": lam.detach().contiguous(),
-                       "A0": A0.detach().contiguous()}
+                       "A0": A0.detach().contiguous(),
+                       "sigma_c": (self.sim.sigma_c if sigma_c is None else sigma_c).detach().contiguous()}
 
     def _step_params(self, k):
         sim = self.sim
@@ -152,7 +157,7 @@ class StepRunner:
         gout = _AdjOut()
         for key in STATE:
             setattr(gout, key, adj_in[key].data_ptr())
-        for key in ("mu", "lam", "A0"):
+        for key in PARAMS:
             setattr(gout, key, grads[key].data_ptr())
         L = _lib_adj()
         _lib.check(L.pmpm_substep_adjoint(C.byref(p), C.byref(self._pin(s_in, bc_id)), C.byref(pfwd), C.byref(gin),
@@ -166,8 +171,8 @@ class StepRunner:
 
 class _Rollout(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, v0, mu, lam, A0, runner, state0, steps, every):
-        runner.set_params(mu, lam, A0)
+    def forward(ctx, v0, mu, lam, A0, sigma_c, runner, state0, steps, every):
+        runner.set_params(mu, lam, A0, sigma_c)
         s = {k: state0[k].detach().contiguous() for k in STATE}
         s["v"] = v0.detach().contiguous()
         ckpt = {0: s}
@@ -195,15 +200,18 @@ class _Rollout(torch.autograd.Function):
             for k in reversed(range(s0, s1)):
                 adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads)
         ctx.adj0 = adj
-        return adj["v"], grads["mu"], grads["lam"], grads["A0"], None, None, None, None
+        return adj["v"], grads["mu"], grads["lam"], grads["A0"], grads["sigma_c"], None, None, None, None
 
 
-def rollout(sim: AnisoMPM, v0, mu, lam, A0, steps: int, every: int = 10):
+def rollout(sim: AnisoMPM, v0, mu, lam, A0, steps: int, every: int = 10, sigma_c=None):
     """Final positions after ``steps`` substeps from ``sim``'s current state,
-    differentiable in v0, mu, lam, A0 (checkpoint every ``every`` steps)."""
+    differentiable in v0, mu, lam, A0 and sigma_c (the damage threshold;
+    default: sim.sigma_c), with a checkpoint every ``every`` steps."""
     runner = StepRunner(sim)
     state0 = {k: getattr(sim, k) for k in STATE}
-    return _Rollout.apply(v0, mu, lam, A0, runner, state0, steps, every)
+    if sigma_c is None:
+        sigma_c = sim.sigma_c
+    return _Rollout.apply(v0, mu, lam, A0, sigma_c, runner, state0, steps, every)
 
 
 def material_tensors(E, nu, a0, alpha):

@@ -1224,7 +1224,7 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
     B.nv = (const T*)fwd->v; B.nC = (const T*)fwd->C;
     B.ox = (T*)gout->x; B.ov = (T*)gout->v; B.oF = (T*)gout->F; B.oC = (T*)gout->C; B.od = (T*)gout->d;
     B.olap = (T*)gout->lap; B.oD = (T*)gout->D_hist; B.omu = (T*)gout->mu; B.olam = (T*)gout->lam;
-    B.oA0 = (T*)gout->A0;
+    B.oA0 = (T*)gout->A0; B.osc = (T*)gout->sigma_c;
     B.gmw = (const T*)aws;
     B.gadj = (T*)aws + 2 * L.G;
     B.pbar = (T*)aws + 8 * L.G;
@@ -1358,7 +1358,7 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
         return fail(PMPM_EARG, "%sthe adjoint supports no f_clamp, no colliders and one scene", "");
     if (in->n > 0) {
         const void* need[] = {gin->x, gin->v, gin->F, gin->C, gin->d, gin->lap, gin->D_hist, gout->x, gout->v, gout->F,
-                              gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, fwd_out->d,
+                              gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, gout->sigma_c, fwd_out->d,
                               fwd_out->D_hist, fwd_out->phi, fwd_out->phi_shear};
         for (const void* q : need)
             if (!q) return fail(PMPM_EARG, "%smissing adjoint array", "");

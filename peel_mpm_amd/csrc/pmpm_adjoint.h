@@ -4,7 +4,8 @@
 //
 // Given the adjoints of the step's outputs (x', v', F', C', d', lap', D'),
 // the backward step produces the adjoints of its inputs (x, v, F, C, d, lap,
-// D_hist) and accumulates those of the per-particle parameters mu, lam, A0.
+// D_hist) and accumulates those of the per-particle parameters mu, lam, A0,
+// sigma_c.
 // The grid of the step is recomputed first (forward binning, P2G and grid
 // pass, with m and sum w exported), then:
 //   k_adj_scatter  G2P adjoint: v_bar and gd_bar onto the grid, one
@@ -26,7 +27,7 @@ template <typename T> struct AdjArgs {
     const T *gx, *gv, *gF, *gC, *gd, *glap, *gD;
     const T *nv, *nC;                   // v', C' of the forward step
     // adjoints written (of the step's inputs) and parameter adjoints (+=)
-    T *ox, *ov, *oF, *oC, *od, *olap, *oD, *omu, *olam, *oA0;
+    T *ox, *ov, *oF, *oC, *od, *olap, *oD, *omu, *olam, *oA0, *osc;
     // grid: forward m, sum w (node-major pairs) and node adjoints (6 per node)
     const T* gmw;
     T* gadj;
@@ -66,7 +67,7 @@ template <typename T>
 __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>& F, bool dmg_on,
                                  const M3<T>& tau_bar, T dn_bar, T Dn_bar,
                                  M3<T>& F_bar, T& d_bar, T& lap_bar, T& Dh_bar, T& mu_bar, T& lam_bar,
-                                 M3<T>& A0_bar) {
+                                 M3<T>& A0_bar, T& sc_bar) {
     // ---- forward
     M3<T> Cm = mul_at(F, F);
     T lc[3];
@@ -157,7 +158,7 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
     T tvol_bar = vdeg * trb;
     const T g_bar = ddot3(tau_bar, tdev) + (tens ? trb * tvol : T(0));
     T dn_tot = dn_bar + g_bar * (T(-2) * (T(1) - dn) * (T(1) - res));
-    d_bar = T(0); lap_bar = T(0); Dh_bar = T(0);
+    d_bar = T(0); lap_bar = T(0); Dh_bar = T(0); sc_bar = T(0);
     T J_bar = T(0);
     M3<T> R_bar;
 #pragma unroll
@@ -183,6 +184,7 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
         max_grad(Dh, Dz, Dn_tot, Dh_bar, Dz_bar);
         const T phi_bar = (phi - T(1)) > T(0) ? A.zeta[p] * Dz_bar : T(0);
         const T k2 = phi_bar * T(2) / (sc * sc);
+        sc_bar = phi_bar * (T(-2) * phi / sc);          // phi = tr(...) / sc^2
         // phi = tr(A s+ A s+)/sc^2:  d/ds+ = 2 A s+ A,  d/dA = 2 s+ A s+
         const M3<T> ASA = mul(mul(Am, sp), Am), SAS = mul(mul(sp, Am), sp);
         M3<T> sp_bar, Am_bar;
@@ -631,9 +633,9 @@ __global__ void __launch_bounds__(WG) k_adj_const(StepArgs<T> A, AdjArgs<T> B) {
         for (int b = 0; b < 3; ++b) tau_bar.m[a][b] = pb[3 * a + b];
     const T dnb_cubic = pb[9];
     M3<T> Fb, A0b;
-    T d_b, lap_b, Dh_b, mu_b, lam_b;
+    T d_b, lap_b, Dh_b, mu_b, lam_b, sc_b;
     const bool dmg_on = B.dmg_on != 0;
-    constitutive_vjp(A, p, F, dmg_on, tau_bar, B.gd[p] + dnb_cubic, B.gD[p], Fb, d_b, lap_b, Dh_b, mu_b, lam_b, A0b);
+    constitutive_vjp(A, p, F, dmg_on, tau_bar, B.gd[p] + dnb_cubic, B.gD[p], Fb, d_b, lap_b, Dh_b, mu_b, lam_b, A0b, sc_b);
     // F' = (I + dt C') F
     const M3<T> Fbn = load_m3(B.gF, p);
     const M3<T> Cn = load_m3(B.nC, p);
@@ -653,6 +655,7 @@ __global__ void __launch_bounds__(WG) k_adj_const(StepArgs<T> A, AdjArgs<T> B) {
     store_m3(B.oF, p, Fb);
     B.omu[p] += mu_b;
     B.olam[p] += lam_b;
+    B.osc[p] += sc_b;
     M3<T> a0 = load_m3((const T*)B.oA0, p);
 #pragma unroll
     for (int a = 0; a < 3; ++a)

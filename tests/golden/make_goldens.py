@@ -338,12 +338,12 @@ def _gold_adjoint(ref):
     n = sim.n
     sim.sigma_c = sim.sigma_c * torch.tensor(rng.uniform(0.5, 40.0, n))   # some particles above the threshold
     leaves = {}
-    for k in ("x", "v", "F", "C", "d", "lap", "D_hist", "mu", "lam", "A0"):
+    for k in ("x", "v", "F", "C", "d", "lap", "D_hist", "mu", "lam", "A0", "sigma_c"):
         t = getattr(sim, k).detach().clone().requires_grad_(True)
         setattr(sim, k, t)
         leaves[k] = t
     rec = {"in_" + k: np_(v) for k, v in leaves.items()}
-    for k in ("vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "allow", "interface_mask", "n_interface"):
+    for k in ("vol0", "mass", "eta", "zeta", "l0", "residual", "allow", "interface_mask", "n_interface"):
         rec["in_" + k] = np_(getattr(sim, k))
     target = leaves["x"].detach() + torch.tensor(rng.normal(0, 1e-3, (n, 3)))
     for s in range(3):
