@@ -26,6 +26,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 
@@ -150,6 +151,35 @@ def time_adjoint(steps, every):
             "grads_finite": finite}
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of ``mib`` MiB, three times: the
+    reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the
+    counters of the step's kernels to bytes (run under rocprofv3 --pmc)."""
+    import torch
+    x = torch.ones(mib * 2 ** 18, device="cuda:0", dtype=torch.float32)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    return {"copy_bytes": mib * 2 ** 20, "repeats": 3}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_config1.md")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled);
+    None when no summary covers them."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s+" + re.escape(workload) + r"\s+" + re.escape(kernel) + r"\s+(\d+)",
+                  text, re.M)
+    return int(m.group(1)) if m else None
+
+
 def time_cpu(spec, steps):
     import torch
     from oracle import cpu_step as O
@@ -178,6 +208,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the step has no multi-GPU path in this build (--gpus {args.gpus}); "
@@ -199,13 +231,16 @@ def main():
         "config": {"workload": f"{args.workload}: {n} particles, {spec['n_grid']}^3 grid, dt={spec['dt']}",
                    "particles": n, "n_grid": spec["n_grid"], "dt": spec["dt"], "form": "plain single-scene"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dominant),
+                     "algorithmic_bytes": alg},
         "kernel_ms": {k: round(v, 5) for k, v in per_kernel.items()},
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
                           "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
         "active_nodes": {"quadratic": n_q, "cubic": n_c},
         "finite": finite,
     }
+    if args.copy_calibration:
+        out["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_batched:
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
     if not args.no_adjoint:
