@@ -1,7 +1,7 @@
 """Benchmark of the MI355X-native AnisoMPM step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
-                    [--cpu-steps 2] [--no-cpu-baseline]
+                    [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
@@ -201,7 +201,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2"))
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-scenes", type=int, default=64)
     ap.add_argument("--batched-steps", type=int, default=20)

@@ -20,7 +20,8 @@ STICKY, SLIP = 0, 1
 
 EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_substep_begin",
            "pmpm_substep_end", "pmpm_grid_views", "pmpm_last_step_stats", "pmpm_bin", "pmpm_stencil",
-           "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read")
+           "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
+           "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint")
 
 
 class Collider(C.Structure):

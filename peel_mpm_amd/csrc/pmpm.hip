@@ -1010,8 +1010,14 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
     }
 }
 
-__global__ void k_zero_u64(unsigned long long* p, int n) {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+// Zero n bytes at a 16-byte aligned p (a fill kernel rather than
+// hipMemsetAsync, so that a captured step holds no memset node).
+__global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p, long long n) {
+    const long long n16 = n >> 4;
+    const long long stride = (long long)gridDim.x * WG;
+    for (long long i = (long long)blockIdx.x * WG + threadIdx.x; i < n16; i += stride)
+        reinterpret_cast<uint4*>(p)[i] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && (long long)threadIdx.x < (n & 15)) p[(n16 << 4) + threadIdx.x] = 0;
 }
 
 #include "pmpm_adjoint.h"
@@ -1172,7 +1178,9 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
     }
-    if (p->export_active) HIPCHK(hipMemsetAsync(ws + L.active, 0, (size_t)L.G, st));
+    if (p->export_active)
+        hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
+                           dim3(WG), 0, st, (unsigned char*)(ws + L.active), (long long)L.G);
     if (in->n > 0) {
         TimedScope ts(1, st);
         if (p->do_damage) {
