@@ -80,7 +80,24 @@ typedef struct {
     int32_t n_scenes;
     const int64_t *scene_offsets;
     const uint8_t *scene_damage;
+    /* Damage update.  PMPM_DAMAGE_EXPLICIT: the reference's explicit update
+     * (anisompm.py:679-685).  PMPM_DAMAGE_PCG: an extension with no
+     * counterpart in the reference -- one backward-Euler step of the same
+     * phase-field equation on the cubic grid, particles as quadrature points,
+     * lumped mass:  (diag(m) + K) d_nodes = b  with
+     *   m_i = sum_p N_ip (1 + c_p (1 + D_p)),  c_p = dt_damage / eta_p,
+     *   K_ij = sum_p c_p l0_p^2 grad N_ip . grad N_jp,  b_i = sum_p N_ip (d_p + c_p D_p),
+     * solved matrix-free by Jacobi-preconditioned CG (D = diag(A)) until the
+     * preconditioned relative residual sqrt(r^T D^-1 r / b^T D^-1 b) is at most
+     * pcg_tol (at most pcg_max_iter iterations), then interpolated to the
+     * particles, clamped to [0,1] and made irreversible as the explicit update
+     * is.  One scene only. */
+    int32_t damage_solver;
+    double pcg_tol;
+    int32_t pcg_max_iter;
 } pmpm_step_params;
+
+enum { PMPM_DAMAGE_EXPLICIT = 0, PMPM_DAMAGE_PCG = 1 };
 
 /* Particle state and materials at the start of the step (AnisoMPM attributes,
  * anisompm.py:374-463). n_interface is read only where interface_mask != 0.
@@ -131,6 +148,12 @@ int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dty
  * the last step on ws; synchronises the stream. */
 int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t n_scenes,
                          int32_t dtype, int64_t *n_active_q, int64_t *n_active_c, void *stream);
+
+/* Iterations and final relative residual sqrt(r^T D^-1 r / b^T D^-1 b) of the most recent PCG
+ * damage solve on ws (zeros before the first one); synchronises the stream.
+ * No counterpart in the reference (its damage update is explicit). */
+int pmpm_damage_solve_info(const void *ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
+                           int32_t *iterations, double *rel_residual, void *stream);
 
 /* ---- adjoint ------------------------------------------------------------- */
 /* Adjoints of a step's outputs (read) ... */

@@ -209,8 +209,11 @@ class OracleMPM:
     class (anisompm.py:303-463) so fixtures and scenes load directly."""
 
     def __init__(self, n_grid=96, grid_lim=1.0, dt=1e-4, gravity=(0.0, -9.8, 0.0),
-                 grid_damp=1.0, f_clamp=None, damage_every=1, dtype=torch.float32):
+                 grid_damp=1.0, f_clamp=None, damage_every=1, dtype=torch.float32, damage_solver="explicit"):
         self.dtype = dtype
+        # "explicit": the reference's update; "implicit": the PCG mode's
+        # backward-Euler grid solve (oracle/pcg_damage.py), solved directly
+        self.damage_solver = damage_solver
         self.n_grid = int(n_grid)
         self.grid_lim = float(grid_lim)
         self.dx = grid_lim / n_grid
@@ -286,6 +289,14 @@ class OracleMPM:
         tn = (Sn * self.n_interface).sum(1, keepdim=True)
         self.phi_shear = ((Sn - tn * self.n_interface).norm(dim=1) / self.sigma_c) ** 2
         self.D_hist = torch.maximum(self.D_hist, self.zeta * torch.relu(phi - 1.0))
+        if self.damage_solver == "implicit":
+            from . import pcg_damage
+            dn, _, _ = pcg_damage.implicit_update(self.x.double().numpy(), self.d.double().numpy(),
+                                                  self.D_hist.double().numpy(), self.eta.double().numpy(),
+                                                  self.l0.double().numpy(), self.allow.numpy(),
+                                                  self.dt * self.damage_every, self.n_grid, self.inv_dx)
+            self.d = torch.from_numpy(dn).to(self.dtype)
+            return
         ddot = torch.relu((1.0 - self.d) * self.D_hist - (self.d - (self.l0 ** 2) * self.lap))
         dnew = torch.clamp(self.d + (self.dt * self.damage_every / self.eta) * ddot, min=0.0, max=1.0)
         self.d = torch.where(self.allow, torch.maximum(dnew, self.d), self.d)

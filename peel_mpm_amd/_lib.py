@@ -17,11 +17,12 @@ PMPM_F32, PMPM_F64 = 0, 1
 MAX_COLLIDERS, MAX_VEL_BC = 8, 8
 SPHERE, HALFSPACE = 1, 2
 STICKY, SLIP = 0, 1
+DAMAGE_EXPLICIT, DAMAGE_PCG = 0, 1
 
 EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_substep_begin",
            "pmpm_substep_end", "pmpm_grid_views", "pmpm_last_step_stats", "pmpm_bin", "pmpm_stencil",
            "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
-           "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint")
+           "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info")
 
 
 class Collider(C.Structure):
@@ -38,7 +39,8 @@ class StepParams(C.Structure):
                 ("n_colliders", C.c_int32), ("colliders", Collider * MAX_COLLIDERS),
                 ("n_vel_bc", C.c_int32), ("vel_bc", (C.c_double * 3) * MAX_VEL_BC),
                 ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32),
-                ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p)]
+                ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p),
+                ("damage_solver", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32)]
 
 
 _IN_PTRS = ("x", "v", "F", "C", "d", "lap", "D_hist", "vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0",
@@ -77,6 +79,7 @@ def lib():
     L.pmpm_bin.argtypes = [vp, i32, i64, i32, dbl, i32, vp, vp, vp, vp, sz, vp]
     L.pmpm_stencil.argtypes = [vp, i32, i64, i32, dbl, vp, vp, vp, vp, vp, vp, vp, vp]
     L.pmpm_sym_eig.argtypes = [vp, i32, i64, vp, vp, vp]
+    L.pmpm_damage_solve_info.argtypes = [vp, i64, i32, i32, i32, C.POINTER(i32), C.POINTER(dbl), vp]
     L.pmpm_last_error.restype = C.c_char_p
     L.pmpm_version.restype = C.c_char_p
     L.pmpm_abi_sizes.argtypes = [C.POINTER(sz), i32]

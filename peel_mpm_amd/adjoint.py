@@ -66,6 +66,8 @@ class StepRunner:
         sim = template
         if sim.f_clamp is not None or sim.colliders or sim.damage_every != 1:
             raise NotImplementedError("the adjoint supports damage_every=1, no f_clamp and no colliders")
+        if getattr(sim, "damage_solver", "explicit") != "explicit":
+            raise NotImplementedError("the adjoint differentiates the explicit damage update only")
         if bool(sim.interface_mask.any()):
             raise NotImplementedError("the adjoint does not differentiate the directional (interface) stress")
         if not all(isinstance(b, VelocityBC) for b in sim.particle_bc):

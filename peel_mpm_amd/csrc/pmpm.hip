@@ -42,6 +42,8 @@ constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket k
 constexpr int P2G_GRID = 2048;         // work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
+// PCG damage scalars (doubles in the pcgr section after the partial dots)
+enum { PCG_RZ = 0, PCG_BB, PCG_ALPHA, PCG_BETA, PCG_REL, PCG_ITER, PCG_DONE, PCG_NSCAL = 8 };
 
 thread_local char g_err[512] = "";
 
@@ -58,8 +60,9 @@ int fail(int code, const char* fmt, const char* a = "", long long b = 0) {
 inline int nblk(int ng) { return (ng + BLK - 1) / BLK; }
 
 // Optional live timing: HIP events recorded on the launch stream around each
-// kernel group, summed on read.  Groups: 0 binning, 1 p2g, 2 grid, 3 g2p.
-constexpr int NTIMED = 4;
+// kernel group, summed on read.  Groups: 0 binning, 1 p2g, 2 grid, 3 g2p,
+// 4 the PCG damage solve (drive, assembly, iterations, interpolation).
+constexpr int NTIMED = 5;
 struct Timing {
     bool on = false;
     std::vector<hipEvent_t> ev[NTIMED];   // start/stop pairs
@@ -101,7 +104,8 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 // holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in
 // active-list order.
 struct Layout {
-    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, stats, total;
+    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, stats, pcgd, pcgr,
+        total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -130,6 +134,8 @@ struct Layout {
         perm = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks
         stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
+        pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
+        pcgr = o;   o = al(o + (size_t)(GRID_WG_COUNT * 3 + PCG_NSCAL) * 8);   // PCG partial dots + scalars
         total = o;
     }
 };
@@ -393,6 +399,8 @@ template <typename T> struct StepArgs {
     // workspace
     T *arena, *gv;
     T* gmw;                     // optional: per node (m, sum w) of the grid pass
+    T* pdiag;                   // PCG damage: per node (Jacobi diagonal, lumped mass)
+    double *ppart, *pscal;      // PCG damage: per-work-group partial dots, scalars
     uint8_t* active;
     const int *start, *alist, *slotof, *blist, *perm, *misc;
     unsigned long long* stats;
@@ -403,6 +411,8 @@ template <typename T> struct StepArgs {
     T dx, inv_dx, dt, lim, pad_lo, pad_hi, fc_lo, fc_hi, dt_dmg, damp, stress_k, c_scale, lap_scale;
     T dtg[3];
     int use_fclamp, export_active, n_col, n_bc;
+    double pcg_tol;
+    int pcg_max;
     pmpm_collider col[PMPM_MAX_COLLIDERS];
     T bcv[PMPM_MAX_VEL_BC][3];
 };
@@ -439,9 +449,12 @@ template <typename T> __device__ __forceinline__ void store_m3(T* __restrict__ a
 
 // Stress, damage and degradation of one particle (anisompm.py:614-706).
 // Returns the degraded Kirchhoff stress; F is replaced by its clamped value
-// when f_clamp is on; d becomes the post-update damage.
-template <typename T, bool DAMAGE, bool EXACT_SIGMA>
-__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J, bool dmg_on) {
+// when f_clamp is on; d becomes the post-update damage.  DRIVE: stop after
+// the drive (phi, phi_shear and the D_hist history max are written; Dh gets
+// the new D_hist) for the implicit damage solve, which updates d itself.
+template <typename T, bool DAMAGE, bool EXACT_SIGMA, bool DRIVE = false>
+__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J, bool dmg_on,
+                                              T* Dh_out = nullptr) {
     M3<T> Cm = mul_at(F, F);
     T lc[3];
     M3<T> V;
@@ -558,6 +571,10 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         // history max of the drive, explicit phase-field update (anisompm.py:679-685)
         const T Dh = maxp(A.D_hist[p], A.zeta[p] * relu(phi - T(1)));
         A.oD[p] = Dh;
+        if constexpr (DRIVE) {
+            *Dh_out = Dh;
+            return M3<T>{};
+        }
         const T l0 = A.l0[p];
         const T resist = d - (l0 * l0) * A.lap[p];
         const T ddot = relu((T(1) - d) * Dh - resist);
@@ -762,6 +779,74 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
     }
 }
 
+// ------------------------------------------------------------------ arena fold
+// Arena slots of the 27 buckets around block (bx,by,bz) (-1: none), fetched
+// together (uniform loads).
+template <typename T>
+__device__ __forceinline__ void bucket_slots(const StepArgs<T>& A, int scene, int bx, int by, int bz, int slot[27]) {
+    const int nb = A.nb;
+    const int* sl = A.slotof + scene * A.NBs;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const int u = bx - 1 + k / 9, v = by - 1 + (k / 3) % 3, w = bz - 1 + k % 3;
+        const bool in = u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
+        slot[k] = in ? sl[(u * nb + v) * nb + w] : -1;
+    }
+}
+
+// Sum of the first NCH channels of node g over the arenas that hold it (the
+// nodes of block b lie in the arenas [4b'-1, 4b'+5] of buckets b' = b-1..b+1
+// per axis), in a fixed bucket order; edge nodes 0 and ng-1 also collect the
+// arena positions beyond the grid (the frames keep those positions
+// unclamped; the reference clamps such nodes onto the edge node).
+template <typename T, int NCH>
+__device__ __forceinline__ void fold_node(const T* __restrict__ arena, const int slot[27], const int g[3], int bx, int by,
+                                          int bz, int ng, T acc[NCH]) {
+    // each node's own position in every arena that holds it: all loads are
+    // independent and issued together
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const int lx = g[0] - (BLK * (bx - 1 + k / 9) - 1);
+        const int ly = g[1] - (BLK * (by - 1 + (k / 3) % 3) - 1);
+        const int lz = g[2] - (BLK * (bz - 1 + k % 3) - 1);
+        if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR) {
+            const T* src = arena + (size_t)slot[k] * (6 * AR3) + (lx * AR + ly) * AR + lz;
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) acc[c] += src[c * AR3];
+        }
+    }
+    const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
+    if (!edge) return;
+    for (int k = 0; k < 27; ++k) {
+        if (slot[k] < 0) continue;
+        const int b3[3] = {bx - 1 + k / 9, by - 1 + (k / 3) % 3, bz - 1 + k % 3};
+        int lo[3], hi[3];
+        bool any_extra = false, ok = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int org = BLK * b3[a] - 1;
+            const int plo = g[a] == 0 ? org : g[a];
+            const int phi = g[a] == ng - 1 ? org + AR - 1 : g[a];
+            lo[a] = (plo > org ? plo : org) - org;
+            hi[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
+            ok = ok && lo[a] <= hi[a];
+            any_extra = any_extra || lo[a] != hi[a] || lo[a] != g[a] - org;
+        }
+        if (!ok || !any_extra) continue;
+        const T* base = arena + (size_t)slot[k] * (6 * AR3);
+        for (int lx = lo[0]; lx <= hi[0]; ++lx)
+            for (int ly = lo[1]; ly <= hi[1]; ++ly)
+                for (int lz = lo[2]; lz <= hi[2]; ++lz) {
+                    // the node's own position was summed above
+                    if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) && lz == g[2] - (BLK * b3[2] - 1))
+                        continue;
+                    const T* src = base + (lx * AR + ly) * AR + lz;
+#pragma unroll
+                    for (int c = 0; c < NCH; ++c) acc[c] += src[c * AR3];
+                }
+    }
+}
+
 // Grid velocity update and colliders per marked 4x4x4 node block
 // (anisompm.py:726-746), cubic normalisation (:777-779).  The nodes of block
 // (bx,by,bz) lie in the arenas [4b-1, 4b+5] of buckets b = bx-1..bx+1 per
@@ -788,63 +873,9 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
         const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
         if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        const int* sl = A.slotof + scene * A.NBs;
-        // slots of the 27 candidate buckets, fetched together (uniform loads)
         int slot[27];
-#pragma unroll
-        for (int k = 0; k < 27; ++k) {
-            const int u = bx - 1 + k / 9, v = by - 1 + (k / 3) % 3, w = bz - 1 + k % 3;
-            const bool in = u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
-            slot[k] = in ? sl[(u * nb + v) * nb + w] : -1;
-        }
-        // each node's own position in every arena that holds it: all loads
-        // are independent and issued together
-#pragma unroll
-        for (int k = 0; k < 27; ++k) {
-            const int lx = g[0] - (BLK * (bx - 1 + k / 9) - 1);
-            const int ly = g[1] - (BLK * (by - 1 + (k / 3) % 3) - 1);
-            const int lz = g[2] - (BLK * (bz - 1 + k % 3) - 1);
-            if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR) {
-                const T* src = A.arena + (size_t)slot[k] * (6 * AR3) + (lx * AR + ly) * AR + lz;
-#pragma unroll
-                for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
-            }
-        }
-        // edge nodes 0 and ng-1 also collect the arena positions beyond the
-        // grid (P2G keeps frame positions unclamped; the reference clamps
-        // them onto the edge node)
-        const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
-        if (edge) {
-            for (int k = 0; k < 27; ++k) {
-                if (slot[k] < 0) continue;
-                const int b3[3] = {bx - 1 + k / 9, by - 1 + (k / 3) % 3, bz - 1 + k % 3};
-                int lo[3], hi[3];
-                bool any_extra = false, ok = true;
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const int org = BLK * b3[a] - 1;
-                    const int plo = g[a] == 0 ? org : g[a];
-                    const int phi = g[a] == ng - 1 ? org + AR - 1 : g[a];
-                    lo[a] = (plo > org ? plo : org) - org;
-                    hi[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
-                    ok = ok && lo[a] <= hi[a];
-                    any_extra = any_extra || lo[a] != hi[a] || lo[a] != g[a] - org;
-                }
-                if (!ok || !any_extra) continue;
-                const T* base = A.arena + (size_t)slot[k] * (6 * AR3);
-                for (int lx = lo[0]; lx <= hi[0]; ++lx)
-                    for (int ly = lo[1]; ly <= hi[1]; ++ly)
-                        for (int lz = lo[2]; lz <= hi[2]; ++lz) {
-                            // the node's own position was summed above
-                            if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) &&
-                                lz == g[2] - (BLK * b3[2] - 1))
-                                continue;
-                            const T* src = base + (lx * AR + ly) * AR + lz;
-#pragma unroll
-                            for (int c = 0; c < 6; ++c) acc[c] += src[c * AR3];
-                        }
-            }
-        }
+        bucket_slots(A, scene, bx, by, bz, slot);
+        fold_node<T, 6>(A.arena, slot, g, bx, by, bz, ng, acc);
         const int gx = g[0], gy = g[1], gz = g[2];
         const long long node = scene * A.Gs + ((long long)gx * ng + gy) * ng + gz;
         const T m = acc[0], mvx = acc[1], mvy = acc[2], mvz = acc[3], dsum = acc[4], wsum = acc[5];
@@ -1010,6 +1041,409 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
     }
 }
 
+// ------------------------------------------------------------------ implicit damage (PCG)
+// One backward-Euler step of  eta d' = (1-d) D - d + l0^2 lap d  on the cubic
+// grid of the reference's Laplacian pass (anisompm.py:762-791), with the
+// particles as quadrature points and the nodal mass lumped:
+//   (diag(m) + K) d_n = b,  m_i = sum_p N_ip (1 + c_p (1 + D_p)),  c_p = dt_damage / eta_p,
+//   K_ij = sum_p c_p l0_p^2 grad N_ip . grad N_jp,  b_i = sum_p N_ip (d_p + c_p D_p),
+// over the nodes with sum_p N_ip > 1e-12.  The reference has no implicit
+// damage solve; this is the build's extension.  Jacobi-preconditioned CG,
+// matrix-free: K p goes through the particles (gather grad p, scatter
+// kappa grad N . grad p) with the P2G arena machinery.  Vectors live per node
+// in gv as (x, r, p, Ap) during the solve (gv is free until the grid pass)
+// and in pdiag as (Jacobi diagonal, lumped mass).  Every kernel of an
+// iteration first reads the done flag, so a converged solve costs one load
+// per remaining launch and the host never waits.
+
+// Cubic B-spline slopes dN/dX (grid units) per axis at the nodes base+0..3.
+template <typename T> __device__ __forceinline__ void cubic_slopes(const T Xp[3], T ds[4][3]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const T f = sub_exact(Xp[a], floor(Xp[a]));
+        const T o = T(1) - f;
+        ds[0][a] = T(-0.5) * (o * o);
+        ds[1][a] = (T(1.5) * f - T(2)) * f;
+        ds[2][a] = (T(2) - T(1.5) * o) * o;
+        ds[3][a] = T(0.5) * (f * f);
+    }
+}
+
+// Work-group sum of NCH doubles, stored as this group's partials
+// (channel-major, GRID_WG_COUNT per channel).  Fixed order: deterministic.
+template <int NCH> __device__ __forceinline__ void store_partials(double v[NCH], double* __restrict__ part) {
+    __shared__ double s[NCH][WG / 64];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+        for (int o = 32; o > 0; o >>= 1) v[c] += __shfl_xor(v[c], o);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) s[c][threadIdx.x >> 6] = v[c];
+    __syncthreads();
+    if (threadIdx.x < NCH) {
+        double t = 0;
+        for (int w = 0; w < WG / 64; ++w) t += s[threadIdx.x][w];
+        part[threadIdx.x * GRID_WG_COUNT + blockIdx.x] = t;
+    }
+}
+
+// Drive + assembly per bucket: the constitutive update up to the D_hist
+// history max (phi, phi_shear, D_hist written), then the cubic scatter of
+// (N mcoef, N bcoef, kappa |grad N|^2, N) into the bucket's arena slot, lane
+// per cell as in P2G.
+template <typename T, bool EXACT_SIGMA>
+__global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
+    constexpr int W = sizeof(T) == 4 ? 4 : 2;
+    __shared__ __align__(16) T s_q[W][AR3][4];
+    const int nact = A.misc[0];
+    const int nb = A.nb, ng = A.ng;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
+        for (int l = threadIdx.x; l < W * 4 * AR3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
+        __syncthreads();
+        const long long cell = ((long long)bk << 6) + lane;
+        const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
+        int rounds = cnt;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        for (int r = wv; r < rounds; r += W) {
+            if (wv >= W || r >= cnt) continue;
+            const long long p = A.perm[c0 + r];
+            M3<T> F = load_m3(A.F, p);
+            T d = A.d[p], J, Dh;
+            constitutive<T, true, EXACT_SIGMA, true>(A, p, F, d, J, true, &Dh);
+            const T c = A.dt_dmg / A.eta[p];
+            const T l0 = A.l0[p];
+            const T kg = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);
+            const T mco = T(1) + c * (T(1) + Dh), bco = d + c * Dh;
+            const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx),
+                             scaled(A.x[3 * p + 2], A.inv_dx)};
+            int bcb[3];
+            T wc[4][3], w2[4][3], ds[4][3];
+            cubic_weights(Xp, ng, bcb, wc, w2);
+            cubic_slopes(Xp, ds);
+            int lp[3][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const T nw = wc[ox][0] * wc[oy][1] * wc[oz][2];
+                        const T gx = ds[ox][0] * wc[oy][1] * wc[oz][2];
+                        const T gy = wc[ox][0] * ds[oy][1] * wc[oz][2];
+                        const T gz = wc[ox][0] * wc[oy][1] * ds[oz][2];
+                        T* q = s_q[wv][lxy + lp[2][oz]];
+                        T q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+                        q0 += nw * mco;
+                        q1 += nw * bco;
+                        q2 += kg * (gx * gx + gy * gy + gz * gz);
+                        q3 += nw;
+                        q[0] = q0; q[1] = q1; q[2] = q2; q[3] = q3;
+                        __asm__ __volatile__("" ::: "memory");
+                    }
+                }
+        }
+        __syncthreads();
+        T* dst = A.arena + (size_t)it.i * (6 * AR3);
+        for (int l = threadIdx.x; l < 4 * AR3; l += P2G_WG) {
+            const int c = l / AR3, node = l - c * AR3;
+            T v = s_q[0][node][c];
+#pragma unroll
+            for (int w = 1; w < W; ++w) v += s_q[w][node][c];
+            dst[l] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// K p through the particles, per bucket: stage the bucket's 343 nodes of the
+// search direction (p_new = r / diag + beta p, or p itself when DIRECT) in
+// LDS, then per particle  g = sum_i grad N_i p_i  and scatter
+// kappa grad N_i . g  into the arena slot (channel 0).
+template <typename T, bool DIRECT>
+__global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
+    if (A.pscal[PCG_DONE] != 0.0) return;
+    constexpr int W = sizeof(T) == 4 ? 4 : 2;
+    __shared__ T s_p[AR3];
+    __shared__ T s_a[W][AR3];
+    const int nact = A.misc[0];
+    const int nb = A.nb, ng = A.ng;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const T beta = T(A.pscal[PCG_BETA]);
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
+        for (int l = threadIdx.x; l < AR3; l += P2G_WG) {
+            const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
+            // arena positions beyond the grid read the edge node (the
+            // reference's clamped node index)
+            const long long node = node_lin(org[0] + lx, org[1] + ly, org[2] + lz, ng);
+            const T* v = A.gv + 4 * node;
+            s_p[l] = DIRECT ? v[2] : v[1] / A.pdiag[2 * node] + beta * v[2];
+        }
+        for (int l = threadIdx.x; l < W * AR3; l += P2G_WG) (&s_a[0][0])[l] = T(0);
+        __syncthreads();
+        const long long cell = ((long long)bk << 6) + lane;
+        const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
+        int rounds = cnt;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        for (int r = wv; r < rounds; r += W) {
+            if (wv >= W || r >= cnt) continue;
+            const long long p = A.perm[c0 + r];
+            const T c = A.dt_dmg / A.eta[p];
+            const T l0 = A.l0[p];
+            const T kg = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);
+            const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx),
+                             scaled(A.x[3 * p + 2], A.inv_dx)};
+            int bcb[3];
+            T wc[4][3], w2[4][3], ds[4][3];
+            cubic_weights(Xp, ng, bcb, wc, w2);
+            cubic_slopes(Xp, ds);
+            int lp[3][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
+            // gather the gradient of p (grid units)
+            T g0 = T(0), g1 = T(0), g2 = T(0);
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const T pv = s_p[lxy + lp[2][oz]];
+                        g0 += (ds[ox][0] * wc[oy][1] * wc[oz][2]) * pv;
+                        g1 += (wc[ox][0] * ds[oy][1] * wc[oz][2]) * pv;
+                        g2 += (wc[ox][0] * wc[oy][1] * ds[oz][2]) * pv;
+                    }
+                }
+            g0 *= kg; g1 *= kg; g2 *= kg;
+#pragma unroll
+            for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy) {
+                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz) {
+                        const T kv = (ds[ox][0] * wc[oy][1] * wc[oz][2]) * g0 + (wc[ox][0] * ds[oy][1] * wc[oz][2]) * g1 +
+                                     (wc[ox][0] * wc[oy][1] * ds[oz][2]) * g2;
+                        T* q = &s_a[wv][lxy + lp[2][oz]];
+                        *q = *q + kv;
+                        __asm__ __volatile__("" ::: "memory");
+                    }
+                }
+        }
+        __syncthreads();
+        T* dst = A.arena + (size_t)it.i * (6 * AR3);
+        for (int l = threadIdx.x; l < AR3; l += P2G_WG) {
+            T v = s_a[0][l];
+#pragma unroll
+            for (int w = 1; w < W; ++w) v += s_a[w][l];
+            dst[l] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// Per marked block, one wavefront (lane = node).  ASSEMBLE: fold the drive's
+// four channels into the Jacobi diagonal, lumped mass, b and x0 = b / m.
+// APPLY / APPLY_DIRECT: fold K p, form Ap = m p + K p on the active nodes,
+// store the search direction and Ap, and the partial dots p.Ap.
+enum { PCG_ASSEMBLE = 0, PCG_APPLY = 1, PCG_APPLY_DIRECT = 2 };
+template <typename T, int MODE>
+__global__ void __launch_bounds__(WG) k_pcg_fold(StepArgs<T> A) {
+    if (MODE != PCG_ASSEMBLE && A.pscal[PCG_DONE] != 0.0) return;
+    constexpr int NCH = MODE == PCG_ASSEMBLE ? 4 : 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
+    const int nwaves = gridDim.x * (WG / 64);
+    const int nblocks = A.misc[1];
+    const int nb = A.nb, ng = A.ng;
+    const T beta = T(A.pscal[PCG_BETA]);
+    double part[1] = {0.0};
+    for (int i = wave; i < nblocks; i += nwaves) {
+        const int blk = A.blist[i];
+        const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+        const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
+        T acc[NCH];
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc[c] = T(0);
+        int slot[27];
+        bucket_slots(A, 0, bx, by, bz, slot);
+        fold_node<T, NCH>(A.arena, slot, g, bx, by, bz, ng, acc);
+        const long long node = ((long long)g[0] * ng + g[1]) * ng + g[2];
+        T* v = A.gv + 4 * node;
+        T* dg = A.pdiag + 2 * node;
+        if constexpr (MODE == PCG_ASSEMBLE) {
+            if (i == 0 && lane == 0) {
+                // a new solve: clear the previous one's flag and scalars (no
+                // kernel of this launch reads them)
+                A.pscal[PCG_DONE] = 0.0; A.pscal[PCG_BETA] = 0.0; A.pscal[PCG_ITER] = 0.0; A.pscal[PCG_REL] = 0.0;
+            }
+            const bool act = acc[3] > T(1e-12);
+            const T m = acc[0], b = acc[1];
+            dg[0] = act ? m + acc[2] : T(1);
+            dg[1] = act ? m : T(0);
+            const T x0 = act ? b / m : T(0);
+            v[0] = x0; v[1] = act ? b : T(0); v[2] = x0; v[3] = T(0);
+        } else {
+            const T m = dg[1];
+            const T pn = MODE == PCG_APPLY_DIRECT ? v[2] : v[1] / dg[0] + beta * v[2];
+            const T ap = m > T(0) ? m * pn + acc[0] : T(0);
+            v[2] = pn;
+            v[3] = ap;
+            part[0] += (double)pn * (double)ap;
+        }
+    }
+    if constexpr (MODE == PCG_APPLY) store_partials<1>(part, A.ppart);
+}
+
+// r = b - A x0 (r holds b, Ap holds A x0), p = 0; partial r.z, r.r, b.D^-1 b.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_pcg_init(StepArgs<T> A) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
+    const int nwaves = gridDim.x * (WG / 64);
+    const int nblocks = A.misc[1];
+    const int nb = A.nb, ng = A.ng;
+    double part[3] = {0.0, 0.0, 0.0};
+    for (int i = wave; i < nblocks; i += nwaves) {
+        const int blk = A.blist[i];
+        const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+        const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
+        const long long node = ((long long)g[0] * ng + g[1]) * ng + g[2];
+        T* v = A.gv + 4 * node;
+        const T b = v[1];
+        const T r = b - v[3];
+        const T dinv = T(1) / A.pdiag[2 * node];
+        v[1] = r;
+        v[2] = T(0);
+        part[0] += (double)r * (double)(r * dinv);
+        part[1] += (double)r * (double)r;
+        part[2] += (double)b * (double)(b * dinv);
+    }
+    store_partials<3>(part, A.ppart);
+}
+
+// x += alpha p, r -= alpha Ap; partial r.z, r.r.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_pcg_update(StepArgs<T> A) {
+    if (A.pscal[PCG_DONE] != 0.0) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
+    const int nwaves = gridDim.x * (WG / 64);
+    const int nblocks = A.misc[1];
+    const int nb = A.nb, ng = A.ng;
+    const T alpha = T(A.pscal[PCG_ALPHA]);
+    double part[2] = {0.0, 0.0};
+    for (int i = wave; i < nblocks; i += nwaves) {
+        const int blk = A.blist[i];
+        const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
+        const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
+        const long long node = ((long long)g[0] * ng + g[1]) * ng + g[2];
+        T* v = A.gv + 4 * node;
+        v[0] = v[0] + alpha * v[2];
+        const T r = v[1] - alpha * v[3];
+        v[1] = r;
+        const T z = r / A.pdiag[2 * node];
+        part[0] += (double)r * (double)z;
+        part[1] += (double)r * (double)r;
+    }
+    store_partials<2>(part, A.ppart);
+}
+
+// One work-group: the partial dots in a fixed order, then the CG scalars.
+enum { PCG_RED_INIT = 0, PCG_RED_ALPHA = 1, PCG_RED_BETA = 2 };
+template <int MODE>
+__global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ part, double* __restrict__ sc, double tol,
+                                                   int max_iter) {
+    if (MODE != PCG_RED_INIT && sc[PCG_DONE] != 0.0) return;
+    constexpr int NCH = MODE == PCG_RED_INIT ? 3 : (MODE == PCG_RED_ALPHA ? 1 : 2);
+    __shared__ double s[NCH][WG];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        double t = 0;
+        for (int i = threadIdx.x; i < GRID_WG_COUNT; i += WG) t += part[c * GRID_WG_COUNT + i];
+        s[c][threadIdx.x] = t;
+    }
+    __syncthreads();
+    for (int h = WG / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int c = 0; c < NCH; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    // convergence is measured on the preconditioned residual,
+    // sqrt(r^T D^-1 r / b^T D^-1 b): it weights every node by its own
+    // diagonal, so nodes with a thin particle support (small lumped mass)
+    // count, where the plain |r| / |b| would let their error through
+    if constexpr (MODE == PCG_RED_INIT) {
+        const double rz = s[0][0], bb = s[2][0];
+        const double rel = bb > 0 ? sqrt(fmax(rz, 0.0) / bb) : 0.0;
+        sc[PCG_RZ] = rz; sc[PCG_BB] = bb; sc[PCG_BETA] = 0.0; sc[PCG_ITER] = 0.0; sc[PCG_REL] = rel;
+        sc[PCG_DONE] = (rel <= tol || max_iter <= 0) ? 1.0 : 0.0;
+    } else if constexpr (MODE == PCG_RED_ALPHA) {
+        const double pap = s[0][0];
+        if (pap > 0) {
+            sc[PCG_ALPHA] = sc[PCG_RZ] / pap;
+        } else {
+            sc[PCG_ALPHA] = 0.0;
+            sc[PCG_DONE] = 1.0;           // breakdown: keep the current iterate
+        }
+    } else {
+        const double rz = s[0][0], bb = sc[PCG_BB];
+        const double it = sc[PCG_ITER] + 1.0;
+        const double rel = bb > 0 ? sqrt(fmax(rz, 0.0) / bb) : 0.0;
+        sc[PCG_BETA] = sc[PCG_RZ] > 0 ? rz / sc[PCG_RZ] : 0.0;
+        sc[PCG_RZ] = rz;
+        sc[PCG_ITER] = it;
+        sc[PCG_REL] = rel;
+        sc[PCG_DONE] = (rel <= tol || it >= (double)max_iter) ? 1.0 : 0.0;
+    }
+}
+
+// New particle damage: the interpolant of the nodal solution, clamped to
+// [0, 1] and made irreversible where allowed, as the explicit update's value
+// is (anisompm.py:684-685).  Particles in sorted order, so neighbours share
+// nodes in the caches.
+template <typename T>
+__global__ void __launch_bounds__(WG) k_pcg_interp(StepArgs<T> A) {
+    const long long i = (long long)blockIdx.x * WG + threadIdx.x;
+    if (i >= A.n) return;
+    const long long p = A.perm[i];
+    const int ng = A.ng;
+    const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx), scaled(A.x[3 * p + 2], A.inv_dx)};
+    int bcb[3];
+    T wc[4][3], w2[4][3];
+    cubic_weights(Xp, ng, bcb, wc, w2);
+    T di = T(0);
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy)
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz)
+                di += (wc[ox][0] * wc[oy][1] * wc[oz][2]) * A.gv[4 * node_lin(bcb[0] + ox, bcb[1] + oy, bcb[2] + oz, ng)];
+    const T dold = A.d[p];
+    const T dn = minp(maxp(di, T(0)), T(1));
+    A.od[p] = A.allow[p] ? maxp(dn, dold) : dold;
+}
+
 // Zero n bytes at a 16-byte aligned p (a fill kernel rather than
 // hipMemsetAsync, so that a captured step holds no memset node).
 __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p, long long n) {
@@ -1113,6 +1547,11 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
     A.misc = (const int*)(ws + L.misc);
     A.stats = (unsigned long long*)(ws + L.stats);
+    A.pdiag = (T*)(ws + L.pcgd);
+    A.ppart = (double*)(ws + L.pcgr);
+    A.pscal = A.ppart + 3 * GRID_WG_COUNT;
+    A.pcg_tol = p->pcg_tol;
+    A.pcg_max = p->pcg_max_iter;
     A.n = in->n;
     A.ng = p->n_grid; A.nb = nblk(p->n_grid);
     A.NBs = L.NBs; A.Gs = L.Gs;
@@ -1143,6 +1582,14 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     if (p->n_colliders < 0 || p->n_colliders > PMPM_MAX_COLLIDERS) return fail(PMPM_EARG, "%stoo many colliders: %lld", "", p->n_colliders);
     if (p->n_vel_bc < 0 || p->n_vel_bc > PMPM_MAX_VEL_BC) return fail(PMPM_EARG, "%stoo many velocity BCs: %lld", "", p->n_vel_bc);
     if (p->n_scenes > 1 && !p->scene_offsets) return fail(PMPM_EARG, "%sbatched step needs scene_offsets", "");
+    if (p->damage_solver != PMPM_DAMAGE_EXPLICIT && p->damage_solver != PMPM_DAMAGE_PCG)
+        return fail(PMPM_EARG, "%sunknown damage_solver %lld", "", p->damage_solver);
+    if (p->damage_solver == PMPM_DAMAGE_PCG) {
+        if (p->n_scenes > 1) return fail(PMPM_EARG, "%sthe PCG damage solve takes one scene", "");
+        if (p->pcg_max_iter < 0 || p->pcg_max_iter > 10000)
+            return fail(PMPM_EARG, "%spcg_max_iter out of range: %lld", "", p->pcg_max_iter);
+        if (!(p->pcg_tol >= 0.0)) return fail(PMPM_EARG, "%spcg_tol must be >= 0", "");
+    }
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     if (in->n > 0) {
@@ -1181,9 +1628,37 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     if (p->export_active)
         hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
                            dim3(WG), 0, st, (unsigned char*)(ws + L.active), (long long)L.G);
+    const bool pcg = p->do_damage && p->damage_solver == PMPM_DAMAGE_PCG && in->n > 0;
+    if (pcg) {
+        TimedScope ts(4, st);
+        const dim3 PG(P2G_GRID), PB(P2G_WG), GG(GRID_WG_COUNT), GB(WG);
+        if (exact_sigma) hipLaunchKernelGGL((k_pcg_drive<T, true>), PG, PB, 0, st, A);
+        else hipLaunchKernelGGL((k_pcg_drive<T, false>), PG, PB, 0, st, A);
+        hipLaunchKernelGGL((k_pcg_fold<T, PCG_ASSEMBLE>), GG, GB, 0, st, A);
+        hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0
+        hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY_DIRECT>), GG, GB, 0, st, A);
+        hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
+        hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal, A.pcg_tol,
+                           A.pcg_max);
+        for (int k = 0; k < p->pcg_max_iter; ++k) {
+            hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
+            hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
+            hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
+                               A.pcg_tol, A.pcg_max);
+            hipLaunchKernelGGL(k_pcg_update<T>, GG, GB, 0, st, A);
+            hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
+                               A.pcg_tol, A.pcg_max);
+        }
+        hipLaunchKernelGGL(k_pcg_interp<T>, dim3(nblocks(in->n)), GB, 0, st, A);
+    }
     if (in->n > 0) {
         TimedScope ts(1, st);
-        if (p->do_damage) {
+        if (pcg) {
+            // the solve wrote d: the transfer uses it (no second update)
+            StepArgs<T> A2 = A;
+            A2.d = A.od;
+            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A2);
+        } else if (p->do_damage) {
             if (exact_sigma) hipLaunchKernelGGL((k_p2g<T, true, true>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
             else hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
         } else {
@@ -1298,6 +1773,19 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
     return PMPM_OK;
 }
 
+int pmpm_damage_solve_info(const void* ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
+                           int32_t* iterations, double* rel_residual, void* stream) {
+    if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
+    Layout L(max_n, n_grid, dtype, n_scenes);
+    double h[PCG_NSCAL];
+    HIPCHK(hipMemcpyAsync(h, (const char*)ws + L.pcgr + (size_t)3 * GRID_WG_COUNT * 8, sizeof h, hipMemcpyDeviceToHost,
+                          (hipStream_t)stream));
+    HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+    if (iterations) *iterations = (int32_t)h[PCG_ITER];
+    if (rel_residual) *rel_residual = h[PCG_REL];
+    return PMPM_OK;
+}
+
 int pmpm_substep_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
                        size_t ws_bytes, void* stream) {
     int64_t n;
@@ -1362,8 +1850,9 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
     if (aws_bytes < pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype))
         return fail(PMPM_ESIZE, "%sadjoint workspace too small, need %lld bytes", "",
                     (long long)pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype));
-    if (p->use_fclamp || p->n_colliders > 0 || p->n_scenes > 1)
-        return fail(PMPM_EARG, "%sthe adjoint supports no f_clamp, no colliders and one scene", "");
+    if (p->use_fclamp || p->n_colliders > 0 || p->n_scenes > 1 || p->damage_solver != PMPM_DAMAGE_EXPLICIT)
+        return fail(PMPM_EARG, "%sthe adjoint supports no f_clamp, no colliders, one scene and the explicit damage update",
+                    "");
     if (in->n > 0) {
         const void* need[] = {gin->x, gin->v, gin->F, gin->C, gin->d, gin->lap, gin->D_hist, gout->x, gout->v, gout->F,
                               gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, gout->sigma_c, fwd_out->d,

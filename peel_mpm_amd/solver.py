@@ -17,7 +17,10 @@ Differences a caller can observe:
   callable is applied between the grid update and G2P on the host;
 * the Cauchy-stress eigenpairs come from the polar stretch's eigenvectors
   (closed form) unless ``exact_sigma_eig=True`` asks for the reference's
-  second Jacobi.  The two agree to float rounding.
+  second Jacobi.  The two agree to float rounding;
+* ``damage_solver="pcg"`` (not in the reference) replaces the explicit
+  damage update by an implicit grid solve (include/pmpm.h,
+  PMPM_DAMAGE_PCG); the default "explicit" is the reference's update.
 """
 from __future__ import annotations
 
@@ -36,7 +39,8 @@ _FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "r
 class AnisoMPM:
     def __init__(self, n_grid: int = 96, grid_lim: float = 1.0, dt: float = 1e-4, gravity=(0.0, -9.8, 0.0),
                  grid_damp: float = 1.0, f_clamp=None, damage_every: int = 1, device: str = "cuda:0",
-                 dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = True):
+                 dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = True,
+                 damage_solver: str = "explicit", pcg_tol: float = 1e-6, pcg_max_iter: int = 50):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the native AnisoMPM step runs on a ROCm GPU (device='cuda:N')")
@@ -54,6 +58,10 @@ class AnisoMPM:
         self.damage_every = max(1, int(damage_every))
         self.exact_sigma_eig = bool(exact_sigma_eig)
         self.stable_sort = bool(stable_sort)
+        if damage_solver not in ("explicit", "pcg"):
+            raise ValueError("damage_solver is 'explicit' (the reference's update) or 'pcg' (implicit grid solve)")
+        self.damage_solver = damage_solver
+        self.pcg_tol, self.pcg_max_iter = float(pcg_tol), int(pcg_max_iter)
         self._step = 0
         r3, r4 = torch.arange(3, device=self.device), torch.arange(4, device=self.device)
         self.offsets = torch.stack([t.reshape(-1) for t in torch.meshgrid(r3, r3, r3, indexing="ij")], 1)
@@ -232,6 +240,8 @@ class AnisoMPM:
             p.vel_bc[k][:] = v
         p.stable_sort = int(self.stable_sort)
         p.exact_sigma_eig = int(self.exact_sigma_eig)
+        p.damage_solver = _lib.DAMAGE_PCG if self.damage_solver == "pcg" else _lib.DAMAGE_EXPLICIT
+        p.pcg_tol, p.pcg_max_iter = self.pcg_tol, self.pcg_max_iter
 
         pin = _lib.ParticlesIn()
         pin.n = n
@@ -294,3 +304,12 @@ class AnisoMPM:
         _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, 1, self._code, C.byref(q),
                                                    C.byref(c), _lib.stream_ptr(self.device)), "pmpm_last_step_stats")
         return q.value, c.value
+
+    def last_damage_solve(self):
+        """(iterations, relative residual) of the most recent PCG damage solve
+        (damage_solver='pcg'); synchronises the stream."""
+        it, rel = C.c_int32(), C.c_double()
+        _lib.check(_lib.lib().pmpm_damage_solve_info(self._ws.data_ptr(), self.n, self.n_grid, 1, self._code,
+                                                     C.byref(it), C.byref(rel), _lib.stream_ptr(self.device)),
+                   "pmpm_damage_solve_info")
+        return it.value, rel.value

@@ -1,0 +1,118 @@
+"""Implicit (PCG) damage mode on the GPU (needs a GPU).
+
+The mode has no counterpart in the reference; its oracle is
+oracle/pcg_damage.py (the same backward-Euler system assembled explicitly
+and solved directly), itself pinned by tests/test_oracle_pcg.py.  Here:
+
+* one step at a time from identical inputs vs the oracle's implicit step:
+  fp64 with the solve run to 1e-12: every field within 1e-9; fp32 at the
+  default tolerance 1e-6: kinematic fields within 1e-5 and damage within
+  2e-5 (the solve's tolerance plus fp32 assembly rounding);
+* the solver's reported residual meets the tolerance, in a few iterations;
+* at the million-particle config1 scene: a uniform damage field under a
+  uniform drive solves to the closed form (d + c D) / (1 + c (1 + D))
+  (K 1 = 0), a size-independent check of the assembly, the fold, the
+  matrix-free operator and the interpolation.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import cpu_step as O
+from tests.golden_cases import rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
+
+
+def damaging_spec(dtype, seed=3):
+    from peel_mpm_amd import scenes
+    spec = scenes.slab_block(n_grid=24, cells=(10, 4, 10), origin=(7, 8, 7), ppc=4, seed=seed, sigma_c=5.0,
+                             pull=(0.0, 2.0, 0.0), dtype=dtype)
+    rng = np.random.default_rng(seed)
+    n = spec["arrays"]["x"].shape[0]
+    spec["arrays"]["d"] = 0.5 * rng.random(n)          # a non-uniform field to diffuse
+    spec["arrays"]["D_hist"] = 20.0 * rng.random(n)
+    return spec
+
+
+def field_err(key, a, b):
+    """Norm-wise relative error; phi (which starts at ~1e-16 at F = I and
+    grows past 1) relative to max(1, max |phi|)."""
+    if key == "phi":
+        b = b.double().cpu()
+        return float((a.double().cpu() - b).abs().max()) / max(1.0, float(b.abs().max()))
+    return rel_err(a, b)
+
+
+SENSITIVE = ("d", "D_hist", "phi", "lap")
+NOISE_X = 3.0
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_pcg_step_matches_the_oracle_solve(dtype):
+    """fp64: every field within 1e-9 of the oracle.  fp32: kinematic fields
+    within 1e-5; the stress-derived fields (and d, which they drive) within
+    NOISE_X times the fp32 oracle's own distance from the fp64 oracle on the
+    same input, plus 2e-6 -- the fp32 noise floor of the small-strain stress,
+    as in the explicit-mode parity tests."""
+    from peel_mpm_amd import AnisoMPM, scenes
+    spec = damaging_spec(dtype)
+    tol = 1e-12 if dtype == torch.float64 else 1e-6
+    # the reference's Jacobi for the Cauchy stress, as the oracle: isolates the solve
+    gpu = scenes.make_solver(AnisoMPM, spec, device=DEV, damage_solver="pcg", pcg_tol=tol, pcg_max_iter=200,
+                             exact_sigma_eig=True)
+    ora = scenes.make_solver(O.OracleMPM, spec, device="cpu", damage_solver="implicit")
+    o64 = scenes.make_solver(O.OracleMPM, dict(spec, dtype=torch.float64), device="cpu", damage_solver="implicit")
+    for s in range(4):
+        for k in STATE:
+            src = getattr(gpu, k).detach().cpu()
+            setattr(ora, k, src.clone())
+            setattr(o64, k, src.double())
+        ora._step = o64._step = gpu._step
+        t = s * spec["dt"]
+        print(f"pcg parity {dtype} step {s}", flush=True)
+        gpu.substep(t)
+        ora.substep(t)
+        if dtype == torch.float32:
+            o64.substep(t)
+        it, rel = gpu.last_damage_solve()
+        errs = {k: field_err(k, getattr(gpu, k), getattr(ora, k)) for k in STATE + ("phi",)}
+        print(f"step {s}: iterations {it} residual {rel:.2e}", {k: f"{e:.1e}" for k, e in errs.items()}, flush=True)
+        assert rel <= tol and 0 <= it <= 50, (it, rel)
+        for k, e in errs.items():
+            if dtype == torch.float64:
+                assert e <= 1e-9, (s, k, e, errs)
+            elif k in SENSITIVE:
+                floor = field_err(k, getattr(ora, k), getattr(o64, k))
+                e64 = field_err(k, getattr(gpu, k), getattr(o64, k))
+                assert e64 <= NOISE_X * floor + 2e-6, (s, k, e64, floor)
+            else:
+                assert e <= 1e-5, (s, k, e, errs)
+    assert float(gpu.d.max()) > float(torch.from_numpy(spec["arrays"]["d"]).max()) - 1e-6
+
+
+def test_pcg_uniform_field_closed_form_at_config1_scale():
+    from peel_mpm_amd import AnisoMPM, scenes
+    spec = scenes.config1()
+    spec["bc"] = []
+    spec["gravity"] = (0.0, 0.0, 0.0)
+    n = spec["arrays"]["x"].shape[0]
+    d0, D0 = 0.2, 50.0
+    spec["arrays"]["d"] = np.full(n, d0)
+    spec["arrays"]["D_hist"] = np.full(n, D0)           # F = I: phi = 0, the drive stays D0
+    spec["arrays"]["eta"] = np.full(n, 0.01)
+    sim = scenes.make_solver(AnisoMPM, spec, device=DEV, damage_solver="pcg")
+    sim.substep(0.0)
+    it, rel = sim.last_damage_solve()
+    c = spec["dt"] / 0.01
+    want = (d0 + c * D0) / (1.0 + c * (1.0 + D0))
+    d = sim.d.double()
+    err = float((d - want).abs().max())
+    print("config1 uniform field:", n, "particles, iterations", it, "residual", rel, "max err", err)
+    assert err <= 2e-6 * want, err
+    assert rel <= 1e-6
+    assert bool(torch.equal(sim.D_hist, torch.full_like(sim.D_hist, D0)))
