@@ -1,7 +1,7 @@
 """Benchmark of the MI355X-native AnisoMPM step.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
-                    [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint]
+                    [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M fp32, batched vs plain; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("binning", "stress_p2g", "grid_update", "g2p")
+PCG_GROUP = 4          # pmpm_timing group of the implicit damage solve
 
 
 def algorithmic_bytes(kernel, n, n_act):
@@ -151,6 +152,36 @@ def time_adjoint(steps, every):
             "grads_finite": finite}
 
 
+def time_pcg(spec, steps, warmup):
+    """The implicit (PCG) damage mode on the same scene: whole-step rate and
+    the solve's own time per step (HIP events around its kernels)."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, _lib, scenes
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", damage_solver="pcg", pcg_tol=1e-6, pcg_max_iter=50)
+    L = _lib.lib()
+    for s in range(warmup):
+        sim.substep(s * spec["dt"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        sim.substep((warmup + s) * spec["dt"])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    it, rel = sim.last_damage_solve()
+    L.pmpm_timing(1)
+    for s in range(steps):
+        sim.substep((warmup + steps + s) * spec["dt"])
+    torch.cuda.synchronize()
+    ms = (C.c_double * 5)()
+    cnt = (C.c_int64 * 5)()
+    _lib.check(L.pmpm_timing_read(ms, cnt, 5), "pmpm_timing_read")
+    L.pmpm_timing(0)
+    return {"workload": "same scene, damage_solver='pcg' (tol 1e-6 on the preconditioned residual, <= 50 iterations)",
+            "value": sim.n * steps / wall, "unit": "particle-steps/s", "ms_per_step": wall / steps * 1e3,
+            "solve_ms_per_step": ms[PCG_GROUP] / max(cnt[PCG_GROUP], 1), "last_iterations": it,
+            "last_residual": rel, "finite": bool(torch.isfinite(sim.d).all())}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, three times: the
     reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the
@@ -208,6 +239,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -241,6 +273,8 @@ def main():
     }
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
+    if not args.no_pcg:
+        out["pcg"] = time_pcg(spec, min(args.steps, 20), 2)
     if not args.no_batched:
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
     if not args.no_adjoint:

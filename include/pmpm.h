@@ -91,7 +91,10 @@ typedef struct {
      * preconditioned relative residual sqrt(r^T D^-1 r / b^T D^-1 b) is at most
      * pcg_tol (at most pcg_max_iter iterations), then interpolated to the
      * particles, clamped to [0,1] and made irreversible as the explicit update
-     * is.  One scene only. */
+     * is.  One scene only.  Outside stream capture the call reads the
+     * solver's convergence flag between chunks of iterations (so it
+     * synchronises the stream a few times); under capture it enqueues all
+     * pcg_max_iter iterations, those after convergence exiting at once. */
     int32_t damage_solver;
     double pcg_tol;
     int32_t pcg_max_iter;

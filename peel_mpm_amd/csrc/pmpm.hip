@@ -7,12 +7,16 @@
 //                 bucket offsets (exclusive scan), ordered list of non-empty
 //                 buckets, marks on the grid blocks their stencils reach
 //   k_bin_scatter perm[start[key]+slot] = particle   (+ k_bin_stabilize)
-//   k_mark_compact list of marked grid blocks (clears the marks)
-//   k_p2g         per non-empty bucket: stress/damage/degradation in
-//                 registers, then quadratic momentum and cubic damage
-//                 scatter into a 7x7x7-node LDS arena (LDS float atomics),
-//                 stored to the bucket's HBM arena slot with plain
-//                 coalesced stores -- no HBM atomics
+//   k_mark_compact list of marked grid blocks (clears the marks); k_inv_perm
+//   k_const       per particle, in particle order: stress/damage/degradation
+//                 in registers, then the particle's P2G record (x, affine
+//                 momentum, mass*v, mass) and d at its sorted position
+//   k_p2g         per non-empty bucket: quadratic momentum and cubic damage
+//                 scatter of the bucket's records into 7x7x7-node LDS arenas
+//                 (lane per cell, plain read-modify-write, two particles per
+//                 update), stored to the bucket's HBM arena slot with plain
+//                 coalesced stores -- no atomics
+//   [PCG damage mode: k_pcg_* between binning and k_const]
 //   k_grid        per marked 4x4x4 node block: each node sums the <= 8
 //                 bucket arenas that cover it in a fixed order, then
 //                 velocity, gravity, damping, colliders, cubic normalisation
@@ -24,6 +28,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/pmpm.h"
@@ -42,6 +47,7 @@ constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket k
 constexpr int P2G_GRID = 2048;         // work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
+constexpr int P2G_REC = 16;            // values per P2G record: x(3), affine(9), mass*v(3), mass
 // PCG damage scalars (doubles in the pcgr section after the partial dots)
 enum { PCG_RZ = 0, PCG_BB, PCG_ALPHA, PCG_BETA, PCG_REL, PCG_ITER, PCG_DONE, PCG_NSCAL = 8 };
 
@@ -104,8 +110,8 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 // holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in
 // active-list order.
 struct Layout {
-    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, misc, stats, pcgd, pcgr,
-        total;
+    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, inv, rec, dsrt, misc, stats,
+        pcgd, pcgr, total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -132,6 +138,9 @@ struct Layout {
         key = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         slot = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         perm = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
+        inv = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);              // sorted position of each particle
+        rec = o;    o = al(o + (size_t)(n > 0 ? n : 1) * P2G_REC * ts);   // P2G records, sorted order
+        dsrt = o;   o = al(o + (size_t)(n > 0 ? n : 1) * ts);             // post-update d, sorted order
         misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks
         stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
@@ -402,7 +411,8 @@ template <typename T> struct StepArgs {
     T* pdiag;                   // PCG damage: per node (Jacobi diagonal, lumped mass)
     double *ppart, *pscal;      // PCG damage: per-work-group partial dots, scalars
     uint8_t* active;
-    const int *start, *alist, *slotof, *blist, *perm, *misc;
+    const int *start, *alist, *slotof, *blist, *perm, *misc, *key, *inv;
+    T *rec, *dsrt;              // P2G records and post-update d at sorted positions
     unsigned long long* stats;
     // scalars
     long long n, NBs, Gs;
@@ -628,15 +638,156 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
     return tau;
 }
 
-// Stress + damage + P2G (quadratic momentum, cubic damage) per bucket.
-// A work-group of P2G_WAVES wavefronts takes one bucket; in every wavefront
-// lane l owns cell l of the bucket, and wavefront w takes rounds
-// w, w + P2G_WAVES, ... (round r = the r-th particle of each cell).  Each
-// wavefront scatters into its own LDS arena; the arenas are summed at the
-// end.  LDS float atomics are avoided altogether (they run at a fraction of
-// the LDS rate even without address conflicts).
-constexpr int P2G_WG = 256;
+// Inverse of the sort permutation: inv[perm[i]] = i.
+__global__ void __launch_bounds__(WG) k_inv_perm(long long n, const int* __restrict__ perm, int* __restrict__ inv) {
+    const long long i = (long long)blockIdx.x * WG + threadIdx.x;
+    if (i < n) inv[perm[i]] = (int)i;
+}
+
+// Stress, damage and degradation per particle, in particle order (the
+// particle arrays are read coalesced), then the particle's P2G record --
+// x, the affine momentum matrix, mass * v (velocity BC applied) and mass
+// (anisompm.py:712-714) -- and its post-update d, stored at the particle's
+// sorted position so that P2G reads a bucket's records contiguously.
 template <typename T, bool DAMAGE, bool EXACT_SIGMA>
+__global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
+    const long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    if (p >= A.n) return;
+    // a batched step gates damage per scene; the scene is in the cell key
+    const bool dmg_on = !DAMAGE || A.scene_dmg == nullptr || A.scene_dmg[(A.key[p] >> 6) / A.NBs] != 0;
+    M3<T> F = load_m3(A.F, p);
+    T d = A.d[p], J;
+    M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J, dmg_on);
+    if (A.use_fclamp) store_m3(A.oF, p, F);
+    store_m3(A.otau, p, tau);
+    A.oJ[p] = J;
+    const T mass = A.mass[p];
+    T sk = T(-1) * A.dt * A.vol0[p];
+    sk = sk * T(4);
+    sk = sk * A.inv_dx;
+    sk = sk * A.inv_dx;
+    const M3<T> Cp = load_m3(A.C, p);
+    // velocity BC slot: compile-time indices only, so the by-value kernel
+    // arguments are never copied to scratch for a dynamic index
+    T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
+    const int bc = A.bc_id ? A.bc_id[p] : 0;
+#pragma unroll
+    for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
+        if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
+    const long long i = A.inv[p];
+    T* r = A.rec + (size_t)P2G_REC * i;
+    r[0] = A.x[3 * p]; r[1] = A.x[3 * p + 1]; r[2] = A.x[3 * p + 2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) r[3 + 3 * a + b] = sk * tau.m[a][b] + mass * Cp.m[a][b];
+    r[12] = mass * vv[0]; r[13] = mass * vv[1]; r[14] = mass * vv[2];
+    r[15] = mass;
+    A.dsrt[i] = d;
+}
+
+// One particle's P2G data in its cell's 4x4x4 frame: weights per axis and
+// frame node (quadratic zero-padded, cubic), node offsets dq (x_i - x_p per
+// axis), affine matrix, mass * v, mass and d.
+template <typename T> struct P2GPart {
+    T wq[4][3], wc[4][3], dq[4][3], af[3][3], mv[3], mass, d;
+    int bcb[3];
+};
+
+template <typename T>
+__device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int ng, P2GPart<T>& q, int sh[3]) {
+    const T* r = A.rec + (size_t)P2G_REC * i;
+    const T Xp[3] = {scaled(r[0], A.inv_dx), scaled(r[1], A.inv_dx), scaled(r[2], A.inv_dx)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) q.af[a][b] = r[3 + 3 * a + b];
+    q.mv[0] = r[12]; q.mv[1] = r[13]; q.mv[2] = r[14];
+    q.mass = r[15];
+    q.d = A.dsrt[i];
+    int bq[3];
+    T fx[3], w[3][3], w2[4][3];
+    quad_weights(Xp, ng, bq, fx, w);
+    cubic_weights(Xp, ng, q.bcb, q.wc, w2);
+    // the quadratic stencil [b, b+2] (b = c-1 or c) sits in the frame at shift sh
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        sh[a] = bq[a] - q.bcb[a];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int j = k - sh[a];
+            q.wq[k][a] = j == 0 ? w[0][a] : (j == 1 ? w[1][a] : (j == 2 ? w[2][a] : T(0)));
+            q.dq[k][a] = (T(j) - fx[a]) * A.dx;
+        }
+    }
+}
+
+// Scatter NP particles that share one frame (lp: arena position of each
+// frame node per axis) into one wavefront's arena: per node the NP
+// contributions are summed in registers, then one read-modify-write of the
+// node's (m, mv) and (sum w d, sum w).  Lanes own distinct cells, so for a
+// fixed node offset the 64 lanes touch 64 distinct entries and no atomics
+// are needed; a compiler barrier after each node keeps the accesses of
+// different lanes to one entry in program order (LDS executes one
+// wavefront's accesses in order).
+template <typename T, int NP>
+__device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp[3][4], const P2GPart<T> (&q)[NP]) {
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox) {
+        T bx[NP][3];
+#pragma unroll
+        for (int j = 0; j < NP; ++j)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bx[j][c] = q[j].mv[c] + q[j].af[c][0] * q[j].dq[ox][0];
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            T wqxy[NP], wcxy[NP], bxy[NP][3];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                wqxy[j] = q[j].wq[ox][0] * q[j].wq[oy][1];
+                wcxy[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) bxy[j][c] = bx[j][c] + q[j].af[c][1] * q[j].dq[oy][1];
+            }
+            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz) {
+                T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), e0 = T(0), e1 = T(0);
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    const T wk = wqxy[j] * q[j].wq[oz][2];
+                    const T wck = wcxy[j] * q[j].wc[oz][2];
+                    a0 += wk * q[j].mass;
+                    a1 += wk * (bxy[j][0] + q[j].af[0][2] * q[j].dq[oz][2]);
+                    a2 += wk * (bxy[j][1] + q[j].af[1][2] * q[j].dq[oz][2]);
+                    a3 += wk * (bxy[j][2] + q[j].af[2][2] * q[j].dq[oz][2]);
+                    e0 += wck * q[j].d;
+                    e1 += wck;
+                }
+                const int l = lxy + lp[2][oz];
+                T* g = sq[l];
+                T c0 = g[0], c1 = g[1], c2 = g[2], c3 = g[3];
+                c0 += a0; c1 += a1; c2 += a2; c3 += a3;
+                g[0] = c0; g[1] = c1; g[2] = c2; g[3] = c3;
+                T* cc = sc[l];
+                T f0 = cc[0], f1 = cc[1];
+                f0 += e0; f1 += e1;
+                cc[0] = f0; cc[1] = f1;
+                __asm__ __volatile__("" ::: "memory");
+            }
+        }
+    }
+}
+
+// P2G (quadratic momentum, cubic damage) per bucket, from the records k_const
+// wrote.  A work-group of P2G_WAVES wavefronts takes one bucket; in every
+// wavefront lane l owns cell l of the bucket and takes two of the cell's
+// particles per pass (wavefront w: particles 2w, 2w+1, then 2w + 2 P2G_WAVES,
+// ...), summed per node in registers before the arena update.  Each
+// wavefront scatters into its own LDS arena; the arenas are summed at the
+// end and stored to the bucket's slot with plain coalesced stores.
+constexpr int P2G_WG = 256;
+template <typename T>
 __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
     constexpr int P2G_WAVES = sizeof(T) == 4 ? 4 : 2;   // fp64 arenas are twice as large
     // per wave: node-major (m, mvx, mvy, mvz) and (sum w d, sum w), so a node's
@@ -648,10 +799,9 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
-        const int scene = (int)(bk / A.NBs), loc = (int)(bk - scene * A.NBs);
+        const int loc = (int)(bk % A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
-        const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
-        const bool dmg_on = A.scene_dmg == nullptr || A.scene_dmg[scene] != 0;
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
         for (int l = threadIdx.x; l < P2G_WAVES * 4 * AR3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
         for (int l = threadIdx.x; l < P2G_WAVES * 2 * AR3; l += P2G_WG) (&s_c[0][0][0])[l] = T(0);
         __syncthreads();
@@ -659,108 +809,42 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
-        for (int r = wv; r < rounds; r += P2G_WAVES) {
+        const int wsel = wv < P2G_WAVES ? wv : 0;
+        for (int r = 2 * wv; r < rounds; r += 2 * P2G_WAVES) {
             if (wv >= P2G_WAVES || r >= cnt) continue;
-            const long long p = A.perm[c0 + r];
-            M3<T> F = load_m3(A.F, p);
-            T d = A.d[p], J;
-            M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J, dmg_on);
-            if (A.use_fclamp) store_m3(A.oF, p, F);
-            store_m3(A.otau, p, tau);
-            A.oJ[p] = J;
-            // affine momentum (anisompm.py:712-714)
-            const T mass = A.mass[p];
-            T sk = T(-1) * A.dt * A.vol0[p];
-            sk = sk * T(4);
-            sk = sk * A.inv_dx;
-            sk = sk * A.inv_dx;
-            M3<T> Cp = load_m3(A.C, p), af;
+            const bool two = r + 1 < cnt;
+            P2GPart<T> q[2];
+            int sh[3];
+            p2g_load(A, c0 + r, ng, q[0], sh);
+            p2g_load(A, c0 + r + (two ? 1 : 0), ng, q[1], sh);
+            // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as
+            // arena positions (not grid-clamped; positions beyond the grid
+            // fold onto the edge node in the grid pass)
+            int lp[3][4];
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
-                for (int b = 0; b < 3; ++b) af.m[a][b] = sk * tau.m[a][b] + mass * Cp.m[a][b];
-            // velocity BC slot: compile-time indices only, so the by-value
-            // kernel arguments are never copied to scratch for a dynamic index
-            T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
-            const int bc = A.bc_id ? A.bc_id[p] : 0;
+                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+            // particles of one cell share the frame unless they lie beyond the
+            // grid (cells clamped together); such a second particle goes alone
+            const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
+            if (!same) {
+                q[1].mass = T(0); q[1].d = T(0);
 #pragma unroll
-            for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
-                if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
-            T mv[3] = {mass * vv[0], mass * vv[1], mass * vv[2]};
-            T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx), scaled(A.x[3 * p + 2], A.inv_dx)};
-            int bq[3], bcb[3];
-            T fx[3], w[3][3], wc[4][3], w2[4][3];
-            quad_weights(Xp, ng, bq, fx, w);
-            cubic_weights(Xp, ng, bcb, wc, w2);
-            // Frame of the particle: the 4 nodes [c-1, c+2] per axis of its
-            // cell c, as arena positions (not grid-clamped; positions beyond
-            // the grid fold onto the edge node in k_grid).  The quadratic
-            // stencil [b, b+2], b = c-1 or c, sits in the frame at shift s.
-            const int org[3] = {o0, o1, o2};
-            int lp[3][4], sh[3];
-            T wq[4][3], dq[4][3];
+                for (int k = 0; k < 4; ++k)
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                sh[a] = bq[a] - bcb[a];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
-                    const int j = k - sh[a];
-                    wq[k][a] = j == 0 ? w[0][a] : (j == 1 ? w[1][a] : (j == 2 ? w[2][a] : T(0)));
-                    dq[k][a] = (T(j) - fx[a]) * A.dx;
-                }
+                    for (int a = 0; a < 3; ++a) { q[1].wq[k][a] = T(0); q[1].wc[k][a] = T(0); }
             }
-            // Scatter with plain LDS read-modify-write: lanes own distinct
-            // cells, so for a fixed (ox, oy, oz) the 64 lanes touch 64
-            // distinct arena entries and no atomics are needed.  Different
-            // iterations of different lanes may touch the same entry: a
-            // compiler barrier after each node keeps the accesses in program
-            // order, and LDS executes one wavefront's accesses in order.
-            const int wsel = wv < P2G_WAVES ? wv : 0;
-            // affine momentum is linear in the node offset: af . dp =
-            // sum over axes of af[:, a] * dq[k_a][a], precomputed per axis
-            T Pa[3][3][4];
+            p2g_scatter<T, 2>(s_q[wsel], s_c[wsel], lp, q);
+            if (two && !same) {
+                P2GPart<T> q1[1];
+                p2g_load(A, c0 + r + 1, ng, q1[0], sh);
+                int lp1[3][4];
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+                for (int a = 0; a < 3; ++a)
 #pragma unroll
-                for (int c = 0; c < 3; ++c)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) Pa[a][c][k] = af.m[c][a] * dq[k][a];
-#pragma unroll
-            for (int ox = 0; ox < 4; ++ox) {
-                T bx[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) bx[c] = mv[c] + Pa[0][c][ox];
-#pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const T wqxy = wq[ox][0] * wq[oy][1];
-                    const T wcxy = wc[ox][0] * wc[oy][1];
-                    T bxy[3];
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) bxy[c] = bx[c] + Pa[1][c][oy];
-                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
-#pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const int l = lxy + lp[2][oz];
-                        const T wk = wqxy * wq[oz][2];
-                        const T wck = wcxy * wc[oz][2];
-                        if (wk != T(0)) {
-                            T* q = s_q[wsel][l];
-                            T c0 = q[0], c1 = q[1], c2 = q[2], c3 = q[3];
-                            c0 += wk * mass;
-                            c1 += wk * (bxy[0] + Pa[2][0][oz]);
-                            c2 += wk * (bxy[1] + Pa[2][1][oz]);
-                            c3 += wk * (bxy[2] + Pa[2][2][oz]);
-                            q[0] = c0; q[1] = c1; q[2] = c2; q[3] = c3;
-                        }
-                        T* cc = s_c[wsel][l];
-                        T e0 = cc[0], e1 = cc[1];
-                        e0 += wck * d;
-                        e1 += wck;
-                        cc[0] = e0; cc[1] = e1;
-                        __asm__ __volatile__("" ::: "memory");
-                    }
-                }
+                    for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+                p2g_scatter<T, 1>(s_q[wsel], s_c[wsel], lp1, q1);
             }
         }
         __syncthreads();
@@ -1499,6 +1583,15 @@ __global__ void __launch_bounds__(WG) k_stencil(const T* __restrict__ x, long lo
 // ------------------------------------------------------------------ host side
 inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
 
+// Constitutive update (particle order) then P2G (bucket order).
+template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
+    const dim3 PB(nblocks(A.n)), B(WG);
+    if (damage && exact_sigma) hipLaunchKernelGGL((k_const<T, true, true>), PB, B, 0, st, A);
+    else if (damage) hipLaunchKernelGGL((k_const<T, true, false>), PB, B, 0, st, A);
+    else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
+    hipLaunchKernelGGL(k_p2g<T>, dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+}
+
 template <typename T>
 int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const long long* scene_off, int ns, const Layout& L,
             char* ws, hipStream_t st) {
@@ -1544,6 +1637,8 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.arena = (T*)(ws + L.arena); A.gv = (T*)(ws + L.gv);
     A.active = (uint8_t*)(ws + L.active);
     A.start = (const int*)(ws + L.start); A.alist = (const int*)(ws + L.alist); A.perm = (const int*)(ws + L.perm);
+    A.key = (const int*)(ws + L.key); A.inv = (const int*)(ws + L.inv);
+    A.rec = (T*)(ws + L.rec); A.dsrt = (T*)(ws + L.dsrt);
     A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
     A.misc = (const int*)(ws + L.misc);
     A.stats = (unsigned long long*)(ws + L.stats);
@@ -1624,6 +1719,9 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         TimedScope ts(0, st);
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
+        if (in->n > 0)
+            hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n,
+                               (const int*)(ws + L.perm), (int*)(ws + L.inv));
     }
     if (p->export_active)
         hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
@@ -1640,14 +1738,32 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
         hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal, A.pcg_tol,
                            A.pcg_max);
-        for (int k = 0; k < p->pcg_max_iter; ++k) {
-            hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
-            hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
-            hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
-                               A.pcg_tol, A.pcg_max);
-            hipLaunchKernelGGL(k_pcg_update<T>, GG, GB, 0, st, A);
-            hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
-                               A.pcg_tol, A.pcg_max);
+        // Iterations go out in chunks of 2, 4, 8, ... with a read of the done
+        // flag between chunks, so a solve that converges in a few iterations
+        // does not pay for pcg_max_iter launches.  Under stream capture no
+        // host read is possible: all pcg_max_iter iterations are enqueued and
+        // the ones after convergence exit at their first load.
+        hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(st, &cap));
+        const bool capturing = cap != hipStreamCaptureStatusNone;
+        int issued = 0, chunk = 2;
+        while (issued < p->pcg_max_iter) {
+            const int k_end = capturing ? p->pcg_max_iter : std::min(p->pcg_max_iter, issued + chunk);
+            for (; issued < k_end; ++issued) {
+                hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
+                hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
+                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
+                                   A.pcg_tol, A.pcg_max);
+                hipLaunchKernelGGL(k_pcg_update<T>, GG, GB, 0, st, A);
+                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
+                                   A.pcg_tol, A.pcg_max);
+            }
+            if (capturing || issued >= p->pcg_max_iter) break;
+            double flag = 0.0;
+            HIPCHK(hipMemcpyAsync(&flag, A.pscal + PCG_DONE, sizeof flag, hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            if (flag != 0.0) break;
+            chunk *= 2;
         }
         hipLaunchKernelGGL(k_pcg_interp<T>, dim3(nblocks(in->n)), GB, 0, st, A);
     }
@@ -1657,12 +1773,9 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             // the solve wrote d: the transfer uses it (no second update)
             StepArgs<T> A2 = A;
             A2.d = A.od;
-            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A2);
-        } else if (p->do_damage) {
-            if (exact_sigma) hipLaunchKernelGGL((k_p2g<T, true, true>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
-            else hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+            launch_transfer<T>(A2, false, false, st);
         } else {
-            hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+            launch_transfer<T>(A, p->do_damage != 0, exact_sigma != 0, st);
         }
     }
     {
@@ -1697,8 +1810,9 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
     hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                        (int*)(ws + L.blist), (int*)(ws + L.misc));
     if (in->n > 0) {
-        if (p->do_damage) hipLaunchKernelGGL((k_p2g<T, true, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
-        else hipLaunchKernelGGL((k_p2g<T, false, false>), dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+        hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n, (const int*)(ws + L.perm),
+                           (int*)(ws + L.inv));
+        launch_transfer<T>(A, p->do_damage != 0, false, st);
     }
     hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     AdjArgs<T> B;
