@@ -49,11 +49,14 @@ def make_scene(name):
     return {"config0": scenes.config0, "config1": scenes.config1, "config2": scenes.config2}[name]()
 
 
-def time_gpu(spec, steps, warmup):
+def time_gpu(spec, steps, warmup, tune=True):
     import torch
-    from peel_mpm_amd import AnisoMPM, _lib, scenes
+    from peel_mpm_amd import AnisoMPM, _lib, autotune, scenes
     sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
     L = _lib.lib()
+    t_tune = time.perf_counter()
+    launch = autotune.tune_solver(sim, steps=2) if tune else autotune.get_config()
+    t_tune = time.perf_counter() - t_tune
     for s in range(warmup):
         sim.substep(s * spec["dt"])
     torch.cuda.synchronize()
@@ -74,15 +77,16 @@ def time_gpu(spec, steps, warmup):
     n_q, n_c = sim.last_step_stats()
     finite = bool(torch.isfinite(sim.x).all())
     per_kernel = {k: (ms[i] / max(cnt[i], 1)) for i, k in enumerate(KERNELS)}
-    return sim.n, wall, per_kernel, (n_q, n_c), finite
+    return sim.n, wall, per_kernel, (n_q, n_c), finite, {"config": launch, "tuning_s": round(t_tune, 2)}
 
 
 def time_batched(n_scenes, steps, warmup):
     """The 64-scene mix: plain AnisoMPM scene by scene vs one batched,
     graph-captured launch set per step."""
     import torch
-    from peel_mpm_amd import AnisoMPM, VelocityBC, scenes
+    from peel_mpm_amd import AnisoMPM, VelocityBC, autotune, scenes
     from peel_mpm_amd.batched import BatchedAnisoMPM
+    autotune.set_config(None)
     specs = [scenes.config3_scene(s) for s in range(n_scenes)]
     total = sum(sp["arrays"]["x"].shape[0] for sp in specs)
     plain = [scenes.make_solver(AnisoMPM, sp, device="cuda:0") for sp in specs]
@@ -102,6 +106,9 @@ def time_batched(n_scenes, steps, warmup):
     for sp in specs:
         b.add_scene(scenes.tensors(sp, "cuda:0"),
                     [VelocityBC(torch.from_numpy(m).to("cuda:0"), tuple(v)) for m, v in sp["bc"]])
+    t_tune = time.perf_counter()
+    launch = autotune.tune_batched(b, steps=2)         # before the graph is captured
+    t_tune = time.perf_counter() - t_tune
     for s in range(warmup):
         b.substep(s * 1e-4)
     torch.cuda.synchronize()
@@ -114,7 +121,8 @@ def time_batched(n_scenes, steps, warmup):
     return {"scenes": n_scenes, "particles": total, "steps": steps,
             "value": total * steps / batched_wall, "plain_scene_by_scene": total * steps / plain_wall,
             "speedup_vs_plain": plain_wall / batched_wall, "ms_per_step": batched_wall / steps * 1e3,
-            "plain_ms_per_step": plain_wall / steps * 1e3, "graph": True, "finite": finite}
+            "plain_ms_per_step": plain_wall / steps * 1e3, "graph": True, "finite": finite,
+            "autotune": {"config": launch, "tuning_s": round(t_tune, 2)}}
 
 
 def time_adjoint(steps, every):
@@ -240,6 +248,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -249,7 +258,7 @@ def main():
         sys.exit(2)
 
     spec = make_scene(args.workload)
-    n, wall, per_kernel, (n_q, n_c), finite = time_gpu(spec, args.steps, args.warmup)
+    n, wall, per_kernel, (n_q, n_c), finite, tuned = time_gpu(spec, args.steps, args.warmup, tune=not args.no_tune)
     value = n * args.steps / wall
     dominant = max(per_kernel, key=per_kernel.get)
     n_act = max(n_q, n_c)
@@ -269,6 +278,7 @@ def main():
         "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
                           "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
         "active_nodes": {"quadratic": n_q, "cubic": n_c},
+        "autotune": tuned,
         "finite": finite,
     }
     if args.copy_calibration:

@@ -158,6 +158,28 @@ int pmpm_last_step_stats(const void *ws, int64_t max_n, int32_t n_grid, int32_t 
 int pmpm_damage_solve_info(const void *ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
                            int32_t *iterations, double *rel_residual, void *stream);
 
+/* ---- launch shape (autotuning) ------------------------------------------ */
+/* Launch shape of the step's kernels, process-wide; zero fields take the
+ * defaults.  No counterpart in the reference (its step is a sequence of
+ * torch ops): these are the knobs the autotuner (peel_mpm_amd/autotune.py)
+ * searches.  A step captured in a HIP graph keeps the shape it was captured
+ * with; set the shape between steps, not while a step is being enqueued on
+ * another thread. */
+typedef struct {
+    int32_t bucket_wgs;     /* work-groups of the persistent per-bucket kernels (P2G, G2P,
+                               PCG drive/apply): a multiple of 8 in [8, 16384]; default 2048 */
+    int32_t block_wgs;      /* work-groups of the per-node-block kernels (grid pass, PCG
+                               fold/update) = the number of PCG partial dot products:
+                               [1, 2048]; default 2048 */
+    int32_t p2g_pair;       /* particles a lane scatters per arena update in P2G: 1 or 2;
+                               default 2 */
+    int32_t const_threads;  /* threads per work-group of the per-particle constitutive
+                               kernel: 64, 128 or 256; default 256 */
+} pmpm_launch_config;
+/* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
+int pmpm_set_launch_config(const pmpm_launch_config *cfg);
+int pmpm_get_launch_config(pmpm_launch_config *cfg);
+
 /* ---- adjoint ------------------------------------------------------------- */
 /* Adjoints of a step's outputs (read) ... */
 typedef struct {

@@ -22,7 +22,8 @@ DAMAGE_EXPLICIT, DAMAGE_PCG = 0, 1
 EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_substep_begin",
            "pmpm_substep_end", "pmpm_grid_views", "pmpm_last_step_stats", "pmpm_bin", "pmpm_stencil",
            "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
-           "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info")
+           "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info",
+           "pmpm_set_launch_config", "pmpm_get_launch_config")
 
 
 class Collider(C.Structure):
@@ -46,6 +47,11 @@ class StepParams(C.Structure):
 _IN_PTRS = ("x", "v", "F", "C", "d", "lap", "D_hist", "vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0",
             "residual", "A0", "allow", "interface_mask", "n_interface", "bc_id")
 _OUT_PTRS = ("x", "v", "F", "C", "lap", "d", "D_hist", "phi", "phi_shear", "tau_last", "J_last")
+
+
+class LaunchConfig(C.Structure):
+    _fields_ = [("bucket_wgs", C.c_int32), ("block_wgs", C.c_int32), ("p2g_pair", C.c_int32),
+                ("const_threads", C.c_int32)]
 
 
 class ParticlesIn(C.Structure):
@@ -80,6 +86,8 @@ def lib():
     L.pmpm_stencil.argtypes = [vp, i32, i64, i32, dbl, vp, vp, vp, vp, vp, vp, vp, vp]
     L.pmpm_sym_eig.argtypes = [vp, i32, i64, vp, vp, vp]
     L.pmpm_damage_solve_info.argtypes = [vp, i64, i32, i32, i32, C.POINTER(i32), C.POINTER(dbl), vp]
+    L.pmpm_set_launch_config.argtypes = [C.POINTER(LaunchConfig)]
+    L.pmpm_get_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.pmpm_last_error.restype = C.c_char_p
     L.pmpm_version.restype = C.c_char_p
     L.pmpm_abi_sizes.argtypes = [C.POINTER(sz), i32]

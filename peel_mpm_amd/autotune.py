@@ -1,0 +1,167 @@
+"""Launch-shape autotuner for the native step.
+
+The step's kernels take their launch shape from a process-wide setting of
+the library (``pmpm_set_launch_config``, include/pmpm.h):
+
+* ``bucket_wgs``: work-groups of the persistent per-bucket kernels (P2G, G2P,
+  the PCG drive and operator) -- how many buckets are in flight per CU;
+* ``block_wgs``: work-groups of the per-node-block kernels (grid pass, PCG
+  fold/update) -- also the number of partial dot products each PCG
+  reduction sums (its reduction shape);
+* ``p2g_pair``: particles a lane accumulates per LDS arena update in P2G
+  (particles per block pass);
+* ``const_threads``: work-group size of the per-particle constitutive
+  kernel.
+
+The bucket itself (4x4x4 cells, one cell per lane of a 64-wide wavefront)
+is the unit of the data layout, not a launch parameter.
+
+``tune`` runs a coordinate descent over these knobs on the caller's own
+workload: for each knob in turn it times a few steps at every value (the
+simulation state is restored before every trial, so tuning leaves the
+simulation where it was) and keeps the fastest.  Results are cached per
+workload shape (solver kind, particle count, grid, scene count, dtype,
+damage solver) for the life of the process.  The reference has no tuner;
+its step is a fixed sequence of torch operations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import torch
+
+from . import _lib
+
+SPACE = {
+    "bucket_wgs": (1024, 2048, 4096),
+    "p2g_pair": (1, 2),
+    "const_threads": (64, 128, 256),
+    "block_wgs": (512, 1024, 2048),
+}
+DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256}
+_CACHE: dict = {}
+
+
+def get_config() -> dict:
+    c = _lib.LaunchConfig()
+    _lib.check(_lib.lib().pmpm_get_launch_config(C.byref(c)), "pmpm_get_launch_config")
+    return {k: getattr(c, k) for k, _ in _lib.LaunchConfig._fields_}
+
+
+def set_config(cfg: dict | None):
+    """Set the launch shape (None: the defaults).  Raises on values the
+    library rejects."""
+    L = _lib.lib()
+    if cfg is None:
+        _lib.check(L.pmpm_set_launch_config(None), "pmpm_set_launch_config")
+        return
+    c = _lib.LaunchConfig(**{k: int(cfg.get(k, 0)) for k, _ in _lib.LaunchConfig._fields_})
+    _lib.check(L.pmpm_set_launch_config(C.byref(c)), "pmpm_set_launch_config")
+
+
+def _time_steps(step, restore, steps, device):
+    restore()
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize(device)
+    return (time.perf_counter() - t0) / steps
+
+
+def tune(step, restore, device, space=None, steps=3, repeats=2, sweeps=1, log=None):
+    """Coordinate descent over ``space`` (default SPACE).  ``step()`` advances
+    the workload one step, ``restore()`` puts its state back.  Returns
+    (best config, {config tuple: seconds per step}) and leaves the best
+    config set."""
+    space = space or SPACE
+    best = dict(DEFAULT)
+    seen = {}
+
+    def measure(cfg):
+        key = tuple(sorted(cfg.items()))
+        if key not in seen:
+            set_config(cfg)
+            step_once = step
+            _time_steps(step_once, restore, 1, device)          # first launch of a new shape
+            seen[key] = min(_time_steps(step_once, restore, steps, device) for _ in range(repeats))
+            if log:
+                log(cfg, seen[key])
+        return seen[key]
+
+    measure(best)
+    for _ in range(sweeps):
+        for knob, values in space.items():
+            for v in values:
+                cand = dict(best, **{knob: v})
+                if measure(cand) < measure(best):
+                    best = cand
+    set_config(best)
+    restore()
+    return best, seen
+
+
+def _solver_key(sim, kind):
+    return (kind, sim.n, sim.n_grid, getattr(sim, "n_scenes", 1), str(sim.dtype),
+            getattr(sim, "damage_solver", "explicit"))
+
+
+def tune_solver(sim, steps=3, space=None, log=None, use_cache=True):
+    """Tune the launch shape on an AnisoMPM's current scene (its state is
+    restored afterwards).  Returns the chosen config."""
+    key = _solver_key(sim, "plain")
+    if use_cache and key in _CACHE:
+        set_config(_CACHE[key])
+        return _CACHE[key]
+    names = ("x", "v", "F", "C", "d", "lap", "D_hist", "phi", "phi_shear", "tau_last", "J_last")
+    saved = {k: getattr(sim, k) for k in names if hasattr(sim, k)}
+    step0 = sim._step
+    t0 = step0 * sim.dt
+
+    def restore():
+        for k, v in saved.items():
+            setattr(sim, k, v)          # the step never writes its inputs: the saved tensors are intact
+        sim._step = step0
+
+    def step():
+        sim.substep(t0)
+
+    best, _ = tune(step, restore, sim.device, space=space, steps=steps, log=log)
+    _CACHE[key] = best
+    return best
+
+
+def tune_batched(b, steps=3, space=None, log=None, use_cache=True):
+    """Tune the launch shape on a BatchedAnisoMPM before its graph is
+    captured (tuning runs the step eagerly; the graph is captured with the
+    chosen shape at the next substep).  Its state is restored afterwards."""
+    if not b._built:
+        b._build()
+    key = _solver_key(b, "batched")
+    if use_cache and key in _CACHE:
+        set_config(_CACHE[key])
+        b._graphs = [None, None]
+        return _CACHE[key]
+    use_graph, b.use_graph = b.use_graph, False
+    cur0, step0 = b._cur, b._step
+    saved = {k: v.clone() for k, v in b._bufs[cur0].items()}
+    t0 = step0 * b.dt
+
+    def restore():
+        b._cur, b._step = cur0, step0
+        for k, v in saved.items():
+            b._bufs[cur0][k].copy_(v)
+        b._bind()
+
+    def step():
+        b.substep(t0)
+
+    try:
+        best, _ = tune(step, restore, b.device, space=space, steps=steps, log=log)
+    finally:
+        b.use_graph = use_graph
+        b._graphs = [None, None]        # recapture with the chosen shape
+        restore()
+    _CACHE[key] = best
+    return best

@@ -104,6 +104,9 @@ struct TimedScope {
 };
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
+// Process-wide launch shape (pmpm_set_launch_config); defaults below.
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256};
+
 // Workspace layout (byte offsets).  The step keeps count and mark at zero
 // between steps: whoever reads them clears them.  count/start are per cell
 // (bucket-major), alist/slotof/mark/blist per bucket or grid block.  arena
@@ -651,7 +654,7 @@ __global__ void __launch_bounds__(WG) k_inv_perm(long long n, const int* __restr
 // sorted position so that P2G reads a bucket's records contiguously.
 template <typename T, bool DAMAGE, bool EXACT_SIGMA>
 __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
-    const long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.n) return;
     // a batched step gates damage per scene; the scene is in the cell key
     const bool dmg_on = !DAMAGE || A.scene_dmg == nullptr || A.scene_dmg[(A.key[p] >> 6) / A.NBs] != 0;
@@ -781,13 +784,14 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 
 // P2G (quadratic momentum, cubic damage) per bucket, from the records k_const
 // wrote.  A work-group of P2G_WAVES wavefronts takes one bucket; in every
-// wavefront lane l owns cell l of the bucket and takes two of the cell's
-// particles per pass (wavefront w: particles 2w, 2w+1, then 2w + 2 P2G_WAVES,
-// ...), summed per node in registers before the arena update.  Each
-// wavefront scatters into its own LDS arena; the arenas are summed at the
-// end and stored to the bucket's slot with plain coalesced stores.
+// wavefront lane l owns cell l of the bucket and takes NP of the cell's
+// particles per pass (wavefront w: particles NP w .. NP w + NP-1, then
+// NP (w + P2G_WAVES), ...), summed per node in registers before the arena
+// update.  Each wavefront scatters into its own LDS arena; the arenas are
+// summed at the end and stored to the bucket's slot with plain coalesced
+// stores.
 constexpr int P2G_WG = 256;
-template <typename T>
+template <typename T, int NP>
 __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
     constexpr int P2G_WAVES = sizeof(T) == 4 ? 4 : 2;   // fp64 arenas are twice as large
     // per wave: node-major (m, mvx, mvy, mvz) and (sum w d, sum w), so a node's
@@ -810,8 +814,19 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
         const int wsel = wv < P2G_WAVES ? wv : 0;
-        for (int r = 2 * wv; r < rounds; r += 2 * P2G_WAVES) {
+        for (int r = NP * wv; r < rounds; r += NP * P2G_WAVES) {
             if (wv >= P2G_WAVES || r >= cnt) continue;
+            if constexpr (NP == 1) {
+                P2GPart<T> q1[1];
+                int sh1[3], lp1[3][4];
+                p2g_load(A, c0 + r, ng, q1[0], sh1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+                p2g_scatter<T, 1>(s_q[wsel], s_c[wsel], lp1, q1);
+                continue;
+            }
             const bool two = r + 1 < cnt;
             P2GPart<T> q[2];
             int sh[3];
@@ -1018,6 +1033,8 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
         for (int w = 0; w < WG / 64; ++w) { q += s_n[0][w]; c += s_n[1][w]; }
         A.stats[2 * blockIdx.x] = q;
         A.stats[2 * blockIdx.x + 1] = c;
+        // how many partials this pass wrote (the launch shape may change)
+        if (blockIdx.x == 0) const_cast<int*>(A.misc)[2] = (int)gridDim.x;
     }
 }
 
@@ -1453,15 +1470,15 @@ __global__ void __launch_bounds__(WG) k_pcg_update(StepArgs<T> A) {
 // One work-group: the partial dots in a fixed order, then the CG scalars.
 enum { PCG_RED_INIT = 0, PCG_RED_ALPHA = 1, PCG_RED_BETA = 2 };
 template <int MODE>
-__global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ part, double* __restrict__ sc, double tol,
-                                                   int max_iter) {
+__global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ part, int nparts, double* __restrict__ sc,
+                                                   double tol, int max_iter) {
     if (MODE != PCG_RED_INIT && sc[PCG_DONE] != 0.0) return;
     constexpr int NCH = MODE == PCG_RED_INIT ? 3 : (MODE == PCG_RED_ALPHA ? 1 : 2);
     __shared__ double s[NCH][WG];
 #pragma unroll
     for (int c = 0; c < NCH; ++c) {
         double t = 0;
-        for (int i = threadIdx.x; i < GRID_WG_COUNT; i += WG) t += part[c * GRID_WG_COUNT + i];
+        for (int i = threadIdx.x; i < nparts; i += WG) t += part[c * GRID_WG_COUNT + i];
         s[c][threadIdx.x] = t;
     }
     __syncthreads();
@@ -1585,11 +1602,13 @@ inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
 
 // Constitutive update (particle order) then P2G (bucket order).
 template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
-    const dim3 PB(nblocks(A.n)), B(WG);
+    const int ct = g_launch.const_threads;
+    const dim3 PB((unsigned)((A.n + ct - 1) / ct)), B(ct);
     if (damage && exact_sigma) hipLaunchKernelGGL((k_const<T, true, true>), PB, B, 0, st, A);
     else if (damage) hipLaunchKernelGGL((k_const<T, true, false>), PB, B, 0, st, A);
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
-    hipLaunchKernelGGL(k_p2g<T>, dim3(P2G_GRID), dim3(P2G_WG), 0, st, A);
+    if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1>), dim3(g_launch.bucket_wgs), dim3(P2G_WG), 0, st, A);
+    else hipLaunchKernelGGL((k_p2g<T, 2>), dim3(g_launch.bucket_wgs), dim3(P2G_WG), 0, st, A);
 }
 
 template <typename T>
@@ -1729,15 +1748,16 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     const bool pcg = p->do_damage && p->damage_solver == PMPM_DAMAGE_PCG && in->n > 0;
     if (pcg) {
         TimedScope ts(4, st);
-        const dim3 PG(P2G_GRID), PB(P2G_WG), GG(GRID_WG_COUNT), GB(WG);
+        const dim3 PG(g_launch.bucket_wgs), PB(P2G_WG), GG(g_launch.block_wgs), GB(WG);
+        const int nparts = g_launch.block_wgs;
         if (exact_sigma) hipLaunchKernelGGL((k_pcg_drive<T, true>), PG, PB, 0, st, A);
         else hipLaunchKernelGGL((k_pcg_drive<T, false>), PG, PB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_ASSEMBLE>), GG, GB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY_DIRECT>), GG, GB, 0, st, A);
         hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
-        hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal, A.pcg_tol,
-                           A.pcg_max);
+        hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts, A.pscal,
+                           A.pcg_tol, A.pcg_max);
         // Iterations go out in chunks of 2, 4, 8, ... with a read of the done
         // flag between chunks, so a solve that converges in a few iterations
         // does not pay for pcg_max_iter launches.  Under stream capture no
@@ -1752,11 +1772,11 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             for (; issued < k_end; ++issued) {
                 hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
                 hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
-                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
-                                   A.pcg_tol, A.pcg_max);
+                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts,
+                                   A.pscal, A.pcg_tol, A.pcg_max);
                 hipLaunchKernelGGL(k_pcg_update<T>, GG, GB, 0, st, A);
-                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, A.pscal,
-                                   A.pcg_tol, A.pcg_max);
+                hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts,
+                                   A.pscal, A.pcg_tol, A.pcg_max);
             }
             if (capturing || issued >= p->pcg_max_iter) break;
             double flag = 0.0;
@@ -1780,7 +1800,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     }
     {
         TimedScope ts(2, st);
-        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
+        hipLaunchKernelGGL(k_grid<T>, dim3(g_launch.block_wgs), dim3(WG), 0, st, A);
     }
     HIPCHK(hipGetLastError());
     return PMPM_OK;
@@ -1792,7 +1812,7 @@ int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
     if (in->n > 0) {
         TimedScope ts(3, st);
-        hipLaunchKernelGGL(k_g2p<T>, dim3(PERSIST_WG), dim3(WG), 0, st, A);
+        hipLaunchKernelGGL(k_g2p<T>, dim3(g_launch.bucket_wgs), dim3(WG), 0, st, A);
     }
     HIPCHK(hipGetLastError());
     return PMPM_OK;
@@ -1878,12 +1898,40 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
                          int64_t* n_active_c, void* stream) {
     Layout L(max_n, n_grid, dtype, n_scenes);
     std::vector<unsigned long long> h(2 * GRID_WG_COUNT);
+    int misc[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(h.data(), (const char*)ws + L.stats, h.size() * sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, (hipStream_t)stream));
+    HIPCHK(hipMemcpyAsync(misc, (const char*)ws + L.misc, sizeof misc, hipMemcpyDeviceToHost, (hipStream_t)stream));
     HIPCHK(hipStreamSynchronize((hipStream_t)stream));
-    for (int i = 1; i < GRID_WG_COUNT; ++i) { h[0] += h[2 * i]; h[1] += h[2 * i + 1]; }
+    const int used = misc[2] > 0 && misc[2] <= GRID_WG_COUNT ? misc[2] : GRID_WG_COUNT;
+    for (int i = 1; i < used; ++i) { h[0] += h[2 * i]; h[1] += h[2 * i + 1]; }
     if (n_active_q) *n_active_q = (int64_t)h[0];
     if (n_active_c) *n_active_c = (int64_t)h[1];
+    return PMPM_OK;
+}
+
+int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256};
+    if (cfg) {
+        if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
+        if (cfg->block_wgs) c.block_wgs = cfg->block_wgs;
+        if (cfg->p2g_pair) c.p2g_pair = cfg->p2g_pair;
+        if (cfg->const_threads) c.const_threads = cfg->const_threads;
+    }
+    if (c.bucket_wgs < 8 || c.bucket_wgs > 16384 || c.bucket_wgs % 8 != 0)
+        return fail(PMPM_EARG, "%sbucket_wgs must be a multiple of 8 in [8, 16384]: %lld", "", c.bucket_wgs);
+    if (c.block_wgs < 1 || c.block_wgs > GRID_WG_COUNT)
+        return fail(PMPM_EARG, "%sblock_wgs must be in [1, 2048]: %lld", "", c.block_wgs);
+    if (c.p2g_pair != 1 && c.p2g_pair != 2) return fail(PMPM_EARG, "%sp2g_pair must be 1 or 2: %lld", "", c.p2g_pair);
+    if (c.const_threads != 64 && c.const_threads != 128 && c.const_threads != 256)
+        return fail(PMPM_EARG, "%sconst_threads must be 64, 128 or 256: %lld", "", c.const_threads);
+    g_launch = c;
+    return PMPM_OK;
+}
+
+int pmpm_get_launch_config(pmpm_launch_config* cfg) {
+    if (!cfg) return fail(PMPM_EARG, "%snull argument", "");
+    *cfg = g_launch;
     return PMPM_OK;
 }
 

@@ -61,3 +61,20 @@ def test_product_refuses_cpu_device():
     from peel_mpm_amd import AnisoMPM
     with pytest.raises(ValueError):
         AnisoMPM(n_grid=16, device="cpu", dtype=torch.float32)
+
+
+def test_launch_config_round_trip_and_validation():
+    """The autotuner's knobs (host-side library state, no kernel runs)."""
+    from peel_mpm_amd import autotune
+    autotune.set_config(None)
+    assert autotune.get_config() == autotune.DEFAULT
+    autotune.set_config({"bucket_wgs": 1024, "block_wgs": 512, "p2g_pair": 1, "const_threads": 128})
+    assert autotune.get_config() == {"bucket_wgs": 1024, "block_wgs": 512, "p2g_pair": 1, "const_threads": 128}
+    for bad in ({"bucket_wgs": 1001}, {"bucket_wgs": 32768}, {"block_wgs": 4096}, {"p2g_pair": 3},
+                {"const_threads": 512}):
+        with pytest.raises(RuntimeError):
+            autotune.set_config(bad)
+    # a rejected setting leaves the previous one in place
+    assert autotune.get_config()["bucket_wgs"] == 1024
+    autotune.set_config(None)
+    assert autotune.get_config() == autotune.DEFAULT

@@ -1,0 +1,91 @@
+"""Launch-shape autotuning (needs a GPU): the knobs change how work is laid
+out on the chip, not the arithmetic -- bucket and block work-group counts
+and the constitutive block size leave the step bit-identical; pairing
+particles in P2G only reorders floating-point sums; tuning leaves the
+simulation's state where it was."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+from tests.golden_cases import rel_err
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
+
+
+def run(cfg, steps=3, **kw):
+    from peel_mpm_amd import AnisoMPM, autotune, scenes
+    autotune.set_config(cfg)
+    spec = scenes.slab_block(n_grid=32, cells=(16, 4, 16), origin=(8, 10, 8), ppc=8, seed=2, sigma_c=20.0)
+    sim = scenes.make_solver(AnisoMPM, spec, device=DEV, **kw)
+    for s in range(steps):
+        sim.substep(s * spec["dt"])
+    torch.cuda.synchronize()
+    autotune.set_config(None)
+    return sim
+
+
+def test_work_group_counts_leave_the_step_bit_identical():
+    ref = run(None)
+    for cfg in ({"bucket_wgs": 1024, "block_wgs": 512}, {"bucket_wgs": 4096, "const_threads": 64},
+                {"block_wgs": 1, "bucket_wgs": 8, "const_threads": 128}):
+        sim = run(cfg)
+        for k in STATE:
+            assert torch.equal(getattr(sim, k), getattr(ref, k)), (cfg, k, rel_err(getattr(sim, k), getattr(ref, k)))
+
+
+def test_single_particle_p2g_matches_paired_to_rounding():
+    a, b = run({"p2g_pair": 2}), run({"p2g_pair": 1})
+    for k in ("x", "v", "F", "C"):
+        assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, k
+
+
+def test_pcg_reduction_shape_changes_only_rounding():
+    a = run({"block_wgs": 2048}, damage_solver="pcg")
+    b = run({"block_wgs": 64}, damage_solver="pcg")
+    for k in ("x", "v", "d"):
+        assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, k
+
+
+def test_tune_solver_restores_state_and_picks_a_valid_config():
+    from peel_mpm_amd import AnisoMPM, autotune, scenes
+    spec = scenes.slab_block(n_grid=32, cells=(16, 4, 16), origin=(8, 10, 8), ppc=8, seed=2)
+    sim = scenes.make_solver(AnisoMPM, spec, device=DEV)
+    sim.substep(0.0)
+    before = {k: getattr(sim, k).clone() for k in STATE}
+    step0 = sim._step
+    small = {"bucket_wgs": (1024, 2048), "p2g_pair": (1, 2), "const_threads": (128, 256), "block_wgs": (1024, 2048)}
+    best = autotune.tune_solver(sim, steps=2, space=small, use_cache=False)
+    assert autotune.get_config() == best
+    assert sim._step == step0
+    for k in STATE:
+        assert torch.equal(getattr(sim, k), before[k]), k
+    sim.substep(step0 * spec["dt"])
+    assert bool(torch.isfinite(sim.x).all())
+    autotune.set_config(None)
+
+
+def test_tune_batched_then_graph_replays_match_eager():
+    from peel_mpm_amd import VelocityBC, autotune, scenes
+    from peel_mpm_amd.batched import BatchedAnisoMPM
+
+    def build(use_graph):
+        b = BatchedAnisoMPM(n_grid=32, dt=1e-4, device=DEV, use_graph=use_graph)
+        for s in range(3):
+            sp = scenes.slab_block(n_grid=32, cells=(6 + 2 * s, 4, 8), origin=(8, 8 + s, 9), ppc=8, seed=s)
+            b.add_scene(scenes.tensors(sp, DEV), [VelocityBC(torch.from_numpy(m).to(DEV), tuple(v)) for m, v in sp["bc"]])
+        return b
+
+    g = build(True)
+    small = {"bucket_wgs": (1024, 2048), "p2g_pair": (2,), "const_threads": (256,), "block_wgs": (1024, 2048)}
+    best = autotune.tune_batched(g, steps=2, space=small, use_cache=False)
+    e = build(False)
+    for s in range(4):
+        g.substep(s * 1e-4)
+        e.substep(s * 1e-4)
+        torch.cuda.synchronize()
+        for k in STATE:
+            assert torch.equal(getattr(g, k), getattr(e, k)), (best, s, k)
+    autotune.set_config(None)
