@@ -98,6 +98,12 @@ typedef struct {
     int32_t damage_solver;
     double pcg_tol;
     int32_t pcg_max_iter;
+    /* Adjoint without recompute.  pmpm_substep with keep_for_adjoint = 1 also
+     * keeps the grid pass's per-node mass sums in ws; pmpm_substep_adjoint
+     * with keep_for_adjoint = 1 then takes binning, P2G records and grid from
+     * ws as that forward call left them (same inputs, same ws, nothing run on
+     * ws in between) instead of recomputing them. */
+    int32_t keep_for_adjoint;
 } pmpm_step_params;
 
 enum { PMPM_DAMAGE_EXPLICIT = 0, PMPM_DAMAGE_PCG = 1 };
@@ -191,13 +197,14 @@ typedef struct {
     void *mu, *lam, *A0, *sigma_c;
 } pmpm_adjoint_out;
 
-/* Bytes of the adjoint's workspace: 8 values per grid node, 10 per particle. */
+/* Bytes of the adjoint's workspace: 6 values per grid node, 10 per particle. */
 size_t pmpm_adjoint_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype);
 /* Backward of one substep (vector-Jacobian product of anisompm.py:593-803).
  * No counterpart in the reference, which has no gradient code.  in = the
  * step's inputs as passed to pmpm_substep; fwd_out = its outputs: v and C
  * are read, d, D_hist, phi, phi_shear, tau_last, J_last are rewritten by the
- * recompute (with the stable sort, to the same values).  p must have
+ * recompute (with the stable sort, to the same values) unless
+ * p->keep_for_adjoint reuses the forward call's workspace.  p must have
  * do_damage as in the forward call.  Supported: no f_clamp, no colliders,
  * one scene, no interface particles (the directional split is not
  * differentiated).  Velocity-BC particles (bc_id) get v_bar = 0. */

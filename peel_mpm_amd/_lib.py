@@ -41,7 +41,8 @@ class StepParams(C.Structure):
                 ("n_vel_bc", C.c_int32), ("vel_bc", (C.c_double * 3) * MAX_VEL_BC),
                 ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32),
                 ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p),
-                ("damage_solver", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32)]
+                ("damage_solver", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32),
+                ("keep_for_adjoint", C.c_int32)]
 
 
 _IN_PTRS = ("x", "v", "F", "C", "d", "lap", "D_hist", "vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0",

@@ -83,6 +83,17 @@ class StepRunner:
         L = _lib_adj()
         nbytes = L.pmpm_adjoint_workspace_bytes(self.n, sim.n_grid, sim._code)
         self._aws = torch.empty(nbytes, dtype=torch.uint8, device=self.dev)
+        self._ring = []             # one step workspace per step of a recomputed segment
+
+    def _ring_ws(self, slot):
+        L = _lib.lib()
+        while len(self._ring) <= slot:
+            nb = L.pmpm_workspace_bytes(self.n, self.sim.n_grid, 1, self.sim._code)
+            w = torch.empty(nb, dtype=torch.uint8, device=self.dev)
+            _lib.check(L.pmpm_workspace_init(w.data_ptr(), nb, self.n, self.sim.n_grid, 1, self.sim._code,
+                                             _lib.stream_ptr(self.dev)), "pmpm_workspace_init")
+            self._ring.append(w)
+        return self._ring[slot]
 
     def set_params(self, mu, lam, A0, sigma_c=None):
         self.params = {"mu": mu.detach().contiguous(), "lam": lam.detach().contiguous(),
@@ -128,25 +139,33 @@ class StepRunner:
                 "phi_shear": torch.empty(n, device=dev, dtype=dt),
                 "tau_last": torch.empty(n, 3, 3, device=dev, dtype=dt), "J_last": torch.empty(n, device=dev, dtype=dt)}
 
-    def step(self, s, k):
-        """State after substep k (time k*dt) from state s."""
+    def step(self, s, k, slot=None):
+        """State after substep k (time k*dt) from state s.  With ``slot`` the
+        step runs on workspace ``slot`` of the segment ring and keeps what
+        its adjoint needs there (step_adjoint with the same slot)."""
         p, bc_id = self._step_params(k)
+        ws = self._ws if slot is None else self._ring_ws(slot)
+        p.keep_for_adjoint = int(slot is not None)
         out = self._scratch_out()
         pout = _lib.ParticlesOut()
         for key, t in out.items():
             setattr(pout, key, t.data_ptr())
         _lib.check(_lib.lib().pmpm_substep(C.byref(p), C.byref(self._pin(s, bc_id)), C.byref(pout),
-                                           self._ws.data_ptr(), self._ws.numel(), _lib.stream_ptr(self.dev)),
-                   "pmpm_substep")
+                                           ws.data_ptr(), ws.numel(), _lib.stream_ptr(self.dev)), "pmpm_substep")
         new = {k2: out[k2] for k2 in STATE}
         if not self.do_damage:
             new["d"], new["D_hist"] = s["d"], s["D_hist"]
         return new
 
-    def step_adjoint(self, s_in, s_out, adj_out, k, grads):
+    def step_adjoint(self, s_in, s_out, adj_out, k, grads, slot=None):
         """Adjoint of substep k: from the adjoints of s_out (adj_out) to those
-        of s_in; parameter adjoints accumulate into grads (mu, lam, A0)."""
+        of s_in; parameter adjoints accumulate into grads (mu, lam, A0,
+        sigma_c).  With ``slot``: the forward step(s_in, k, slot) was the last
+        thing run on that workspace, and its binning and grid are reused
+        instead of recomputed."""
         p, bc_id = self._step_params(k)
+        ws = self._ws if slot is None else self._ring_ws(slot)
+        p.keep_for_adjoint = int(slot is not None)
         scratch = self._scratch_out()
         scratch["v"], scratch["C"] = s_out["v"], s_out["C"]
         pfwd = _lib.ParticlesOut()
@@ -163,7 +182,7 @@ class StepRunner:
             setattr(gout, key, grads[key].data_ptr())
         L = _lib_adj()
         _lib.check(L.pmpm_substep_adjoint(C.byref(p), C.byref(self._pin(s_in, bc_id)), C.byref(pfwd), C.byref(gin),
-                                          C.byref(gout), self._ws.data_ptr(), self._ws.numel(), self._aws.data_ptr(),
+                                          C.byref(gout), ws.data_ptr(), ws.numel(), self._aws.data_ptr(),
                                           self._aws.numel(), _lib.stream_ptr(self.dev)), "pmpm_substep_adjoint")
         return adj_in
 
@@ -198,9 +217,9 @@ class _Rollout(torch.autograd.Function):
             s1 = min(s0 + every, steps)
             states = [ctx.ckpt[s0]]
             for k in range(s0, s1):
-                states.append(runner.step(states[-1], k))
+                states.append(runner.step(states[-1], k, slot=k - s0))
             for k in reversed(range(s0, s1)):
-                adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads)
+                adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads, slot=k - s0)
         ctx.adj0 = adj
         return adj["v"], grads["mu"], grads["lam"], grads["A0"], grads["sigma_c"], None, None, None, None
 
@@ -226,9 +245,11 @@ def material_tensors(E, nu, a0, alpha):
     return mu, lam, A0
 
 
-def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10):
-    """Adjoint of the whole initial state and of mu, lam, A0 for the vector
-    x_bar on the positions after ``steps`` substeps (no autograd graph)."""
+def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10, reuse: bool = True):
+    """Adjoint of the whole initial state and of mu, lam, A0, sigma_c for the
+    vector x_bar on the positions after ``steps`` substeps (no autograd
+    graph).  reuse=False makes every adjoint step recompute its forward
+    binning and grid instead of reusing the segment's workspaces."""
     runner = StepRunner(sim)
     s = {k: getattr(sim, k).contiguous() for k in STATE}
     ckpt = {0: s}
@@ -243,7 +264,8 @@ def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10):
         s1 = min(s0 + every, steps)
         states = [ckpt[s0]]
         for k in range(s0, s1):
-            states.append(runner.step(states[-1], k))
+            states.append(runner.step(states[-1], k, slot=(k - s0) if reuse else None))
         for k in reversed(range(s0, s1)):
-            adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads)
+            adj = runner.step_adjoint(states[k - s0], states[k - s0 + 1], adj, k, grads,
+                                      slot=(k - s0) if reuse else None)
     return s, adj, grads

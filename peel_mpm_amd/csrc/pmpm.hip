@@ -114,7 +114,7 @@ pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256};
 // active-list order.
 struct Layout {
     size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, inv, rec, dsrt, misc, stats,
-        pcgd, pcgr, total;
+        pcgd, pcgr, gmw, total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -147,6 +147,7 @@ struct Layout {
         misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks
         stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
+        gmw = o;    o = al(o + (size_t)G * 2 * ts);                // per node (m, sum w) kept for the adjoint
         pcgr = o;   o = al(o + (size_t)(GRID_WG_COUNT * 3 + PCG_NSCAL) * 8);   // PCG partial dots + scalars
         total = o;
     }
@@ -1661,6 +1662,7 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
     A.misc = (const int*)(ws + L.misc);
     A.stats = (unsigned long long*)(ws + L.stats);
+    A.gmw = p->keep_for_adjoint ? (T*)(ws + L.gmw) : nullptr;
     A.pdiag = (T*)(ws + L.pcgd);
     A.ppart = (double*)(ws + L.pcgr);
     A.pscal = A.ppart + 3 * GRID_WG_COUNT;
@@ -1822,19 +1824,21 @@ template <typename T>
 int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* fwd,
                  const pmpm_adjoint_in* gin, const pmpm_adjoint_out* gout, char* ws, char* aws, hipStream_t st) {
     Layout L(in->n, p->n_grid, p->dtype, 1);
-    // recompute the step's binning, P2G and grid pass, exporting (m, sum w)
-    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, 1, nullptr, 1, L, ws, st);
-    if (rc) return rc;
     StepArgs<T> A = make_args<T>(p, in, fwd, L, ws);
-    A.gmw = (T*)aws;
-    hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
-                       (int*)(ws + L.blist), (int*)(ws + L.misc));
-    if (in->n > 0) {
-        hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n, (const int*)(ws + L.perm),
-                           (int*)(ws + L.inv));
-        launch_transfer<T>(A, p->do_damage != 0, false, st);
+    A.gmw = (T*)(ws + L.gmw);
+    if (!p->keep_for_adjoint) {
+        // recompute the step's binning, P2G and grid pass, keeping (m, sum w)
+        int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, 1, nullptr, 1, L, ws, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
+                           (int*)(ws + L.blist), (int*)(ws + L.misc));
+        if (in->n > 0) {
+            hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n,
+                               (const int*)(ws + L.perm), (int*)(ws + L.inv));
+            launch_transfer<T>(A, p->do_damage != 0, false, st);
+        }
+        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     }
-    hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     AdjArgs<T> B;
     B.gx = (const T*)gin->x; B.gv = (const T*)gin->v; B.gF = (const T*)gin->F; B.gC = (const T*)gin->C;
     B.gd = (const T*)gin->d; B.glap = (const T*)gin->lap; B.gD = (const T*)gin->D_hist;
@@ -1842,9 +1846,9 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
     B.ox = (T*)gout->x; B.ov = (T*)gout->v; B.oF = (T*)gout->F; B.oC = (T*)gout->C; B.od = (T*)gout->d;
     B.olap = (T*)gout->lap; B.oD = (T*)gout->D_hist; B.omu = (T*)gout->mu; B.olam = (T*)gout->lam;
     B.oA0 = (T*)gout->A0; B.osc = (T*)gout->sigma_c;
-    B.gmw = (const T*)aws;
-    B.gadj = (T*)aws + 2 * L.G;
-    B.pbar = (T*)aws + 8 * L.G;
+    B.gmw = A.gmw;
+    B.gadj = (T*)aws;
+    B.pbar = (T*)aws + 6 * L.G;
     B.dmg_on = p->do_damage;
     if (in->n > 0) {
         hipLaunchKernelGGL(k_adj_scatter<T>, dim3(P2G_GRID), dim3(P2G_WG), 0, st, A, B);
@@ -1999,7 +2003,7 @@ int pmpm_timing_read(double* total_ms, int64_t* launches, int32_t n) {
 size_t pmpm_adjoint_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype) {
     if (n_grid < 1 || max_n < 0) return 0;
     const size_t ts = dtype == PMPM_F64 ? 8 : 4;
-    return ((size_t)n_grid * n_grid * n_grid * 8 + (size_t)max_n * 10) * ts;
+    return ((size_t)n_grid * n_grid * n_grid * 6 + (size_t)max_n * 10) * ts;
 }
 
 int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* fwd_out,

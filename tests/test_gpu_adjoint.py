@@ -157,6 +157,23 @@ def test_adjoint_vs_central_finite_differences_damage(which):
     assert abs(an - fd) <= 1e-4 * abs(fd), (which, an, fd)
 
 
+def test_reused_forward_workspaces_give_the_recomputed_gradients():
+    """The backward reuses each segment step's binning and grid (one
+    workspace per step); recomputing them inside every adjoint step instead
+    must give bit-identical gradients (the stable sort makes the recompute
+    exact)."""
+    from peel_mpm_amd.adjoint import state_vjp
+    sim, z, steps = sim_from_golden()
+    target = torch.from_numpy(z["target"]).to(DEV)
+    xb = 2.0 * (sim.x - target)
+    _, a1, g1 = state_vjp(sim, steps, xb, every=2, reuse=True)
+    _, a2, g2 = state_vjp(sim, steps, xb, every=2, reuse=False)
+    for k in STATE:
+        assert torch.equal(a1[k], a2[k]), k
+    for k in g1:
+        assert torch.equal(g1[k], g2[k]), k
+
+
 def test_checkpoint_spacing_does_not_change_gradients():
     from peel_mpm_amd.adjoint import material_tensors, rollout
     steps = 12
