@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
                     [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
+                    [--no-config2] [--no-tune]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
@@ -249,6 +250,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -285,6 +287,16 @@ def main():
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_pcg:
         out["pcg"] = time_pcg(spec, min(args.steps, 20), 2)
+    if not args.no_config2 and args.workload != "config2":
+        import torch
+        s2 = make_scene("config2")
+        n2, wall2, pk2, (q2, c2), fin2, tuned2 = time_gpu(s2, 10, 2, tune=not args.no_tune)
+        out["config2"] = {"workload": f"config2: {n2} particles, 256^3 grid, two slabs, dt=2e-05", "value": n2 * 10 / wall2,
+                          "unit": "particle-steps/s", "ms_per_step": wall2 / 10 * 1e3,
+                          "kernel_ms": {k: round(v, 5) for k, v in pk2.items()},
+                          "active_nodes": {"quadratic": q2, "cubic": c2}, "finite": fin2, "autotune": tuned2}
+        del s2
+        torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
     if not args.no_adjoint:
