@@ -866,10 +866,10 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
         __syncthreads();
         // sum the wavefront arenas in a fixed order and store this bucket's
         // slot with plain coalesced stores
-        // slot layout: channel-major [6][343] (m, mvx, mvy, mvz, sum w d, sum w)
+        // slot layout: node-major [343][6] (m, mvx, mvy, mvz, sum w d, sum w)
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < 6 * AR3; l += P2G_WG) {
-            const int c = l / AR3, node = l - c * AR3;
+            const int node = l / 6, c = l - node * 6;
             T v = c < 4 ? s_q[0][node][c] : s_c[0][node][c - 4];
 #pragma unroll
             for (int w = 1; w < P2G_WAVES; ++w) v += c < 4 ? s_q[w][node][c] : s_c[w][node][c - 4];
@@ -894,6 +894,23 @@ __device__ __forceinline__ void bucket_slots(const StepArgs<T>& A, int scene, in
     }
 }
 
+// acc[c] += the node's channel c, c < NCH, from its 6 node-major values
+// (8-byte pairs in fp32: a node's values start 24-byte aligned).
+template <typename T, int NCH> __device__ __forceinline__ void add_node(const T* __restrict__ src, T acc[NCH]) {
+    if constexpr (sizeof(T) == 4 && NCH % 2 == 0) {
+        const float2* s2 = reinterpret_cast<const float2*>(src);
+#pragma unroll
+        for (int c = 0; c < NCH / 2; ++c) {
+            const float2 v = s2[c];
+            acc[2 * c] += v.x;
+            acc[2 * c + 1] += v.y;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) acc[c] += src[c];
+    }
+}
+
 // Sum of the first NCH channels of node g over the arenas that hold it (the
 // nodes of block b lie in the arenas [4b'-1, 4b'+5] of buckets b' = b-1..b+1
 // per axis), in a fixed bucket order; edge nodes 0 and ng-1 also collect the
@@ -909,11 +926,8 @@ __device__ __forceinline__ void fold_node(const T* __restrict__ arena, const int
         const int lx = g[0] - (BLK * (bx - 1 + k / 9) - 1);
         const int ly = g[1] - (BLK * (by - 1 + (k / 3) % 3) - 1);
         const int lz = g[2] - (BLK * (bz - 1 + k % 3) - 1);
-        if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR) {
-            const T* src = arena + (size_t)slot[k] * (6 * AR3) + (lx * AR + ly) * AR + lz;
-#pragma unroll
-            for (int c = 0; c < NCH; ++c) acc[c] += src[c * AR3];
-        }
+        if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR)
+            add_node<T, NCH>(arena + (size_t)slot[k] * (6 * AR3) + 6 * ((lx * AR + ly) * AR + lz), acc);
     }
     const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
     if (!edge) return;
@@ -940,9 +954,7 @@ __device__ __forceinline__ void fold_node(const T* __restrict__ arena, const int
                     // the node's own position was summed above
                     if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) && lz == g[2] - (BLK * b3[2] - 1))
                         continue;
-                    const T* src = base + (lx * AR + ly) * AR + lz;
-#pragma unroll
-                    for (int c = 0; c < NCH; ++c) acc[c] += src[c * AR3];
+                    add_node<T, NCH>(base + 6 * ((lx * AR + ly) * AR + lz), acc);
                 }
     }
 }
@@ -1256,11 +1268,11 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
         __syncthreads();
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < 4 * AR3; l += P2G_WG) {
-            const int c = l / AR3, node = l - c * AR3;
+            const int node = l / 4, c = l - node * 4;
             T v = s_q[0][node][c];
 #pragma unroll
             for (int w = 1; w < W; ++w) v += s_q[w][node][c];
-            dst[l] = v;
+            dst[6 * node + c] = v;
         }
         __syncthreads();
     }
@@ -1352,7 +1364,7 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
             T v = s_a[0][l];
 #pragma unroll
             for (int w = 1; w < W; ++w) v += s_a[w][l];
-            dst[l] = v;
+            dst[6 * l] = v;
         }
         __syncthreads();
     }
