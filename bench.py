@@ -251,6 +251,8 @@ def main():
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
+    ap.add_argument("--launch-config", default=None, metavar="JSON",
+                    help="fixed launch shape, e.g. '{\"bucket_wgs\": 4096}' (implies --no-tune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -259,6 +261,10 @@ def main():
               "scenes shard across GPUs as independent processes", file=sys.stderr)
         sys.exit(2)
 
+    if args.launch_config:
+        from peel_mpm_amd import autotune
+        autotune.set_config(json.loads(args.launch_config))
+        args.no_tune = True
     spec = make_scene(args.workload)
     n, wall, per_kernel, (n_q, n_c), finite, tuned = time_gpu(spec, args.steps, args.warmup, tune=not args.no_tune)
     value = n * args.steps / wall
