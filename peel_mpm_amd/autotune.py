@@ -10,6 +10,9 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   reduction sums (its reduction shape);
 * ``p2g_pair``: particles a lane accumulates per LDS arena update in P2G
   (particles per block pass);
+* ``p2g_waves``: wavefronts per bucket in P2G, each with its own LDS arena
+  (2: smaller work-groups, more of them per CU; 4: more particles of a
+  bucket in flight);
 * ``const_threads``: work-group size of the per-particle constitutive
   kernel.
 
@@ -34,12 +37,13 @@ import torch
 from . import _lib
 
 SPACE = {
-    "bucket_wgs": (1024, 2048, 4096),
+    "bucket_wgs": (2048, 4096, 8192),
+    "p2g_waves": (2, 4),
     "p2g_pair": (1, 2),
     "const_threads": (64, 128, 256),
     "block_wgs": (512, 1024, 2048),
 }
-DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256}
+DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2}
 _CACHE: dict = {}
 
 

@@ -105,7 +105,7 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
-pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256};
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2};
 
 // Workspace layout (byte offsets).  The step keeps count and mark at zero
 // between steps: whoever reads them clears them.  count/start are per cell
@@ -784,7 +784,7 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 }
 
 // P2G (quadratic momentum, cubic damage) per bucket, from the records k_const
-// wrote.  A work-group of P2G_WAVES wavefronts takes one bucket; in every
+// wrote.  A work-group of P2G_WAVES (2 or 4) wavefronts takes one bucket; in every
 // wavefront lane l owns cell l of the bucket and takes NP of the cell's
 // particles per pass (wavefront w: particles NP w .. NP w + NP-1, then
 // NP (w + P2G_WAVES), ...), summed per node in registers before the arena
@@ -792,9 +792,9 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 // summed at the end and stored to the bucket's slot with plain coalesced
 // stores.
 constexpr int P2G_WG = 256;
-template <typename T, int NP>
-__global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
-    constexpr int P2G_WAVES = sizeof(T) == 4 ? 4 : 2;   // fp64 arenas are twice as large
+template <typename T, int NP, int P2G_WAVES>
+__global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g(StepArgs<T> A) {
+    constexpr int NT = 64 * P2G_WAVES;
     // per wave: node-major (m, mvx, mvy, mvz) and (sum w d, sum w), so a node's
     // update is one 16-byte and one 8-byte LDS access (fp32)
     __shared__ __align__(16) T s_q[P2G_WAVES][AR3][4];
@@ -807,8 +807,8 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
         const int loc = (int)(bk % A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
-        for (int l = threadIdx.x; l < P2G_WAVES * 4 * AR3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
-        for (int l = threadIdx.x; l < P2G_WAVES * 2 * AR3; l += P2G_WG) (&s_c[0][0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < P2G_WAVES * 4 * AR3; l += NT) (&s_q[0][0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < P2G_WAVES * 2 * AR3; l += NT) (&s_c[0][0][0])[l] = T(0);
         __syncthreads();
         const long long cell = ((long long)bk << 6) + lane;
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
@@ -868,7 +868,7 @@ __global__ void __launch_bounds__(P2G_WG) k_p2g(StepArgs<T> A) {
         // slot with plain coalesced stores
         // slot layout: node-major [343][6] (m, mvx, mvy, mvz, sum w d, sum w)
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
-        for (int l = threadIdx.x; l < 6 * AR3; l += P2G_WG) {
+        for (int l = threadIdx.x; l < 6 * AR3; l += NT) {
             const int node = l / 6, c = l - node * 6;
             T v = c < 4 ? s_q[0][node][c] : s_c[0][node][c - 4];
 #pragma unroll
@@ -1620,8 +1620,19 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
     if (damage && exact_sigma) hipLaunchKernelGGL((k_const<T, true, true>), PB, B, 0, st, A);
     else if (damage) hipLaunchKernelGGL((k_const<T, true, false>), PB, B, 0, st, A);
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
-    if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1>), dim3(g_launch.bucket_wgs), dim3(P2G_WG), 0, st, A);
-    else hipLaunchKernelGGL((k_p2g<T, 2>), dim3(g_launch.bucket_wgs), dim3(P2G_WG), 0, st, A);
+    const dim3 G(g_launch.bucket_wgs);
+    // four fp64 arenas would exceed 64 KiB of LDS per work-group: fp64 always takes two
+    if constexpr (sizeof(T) == 4) {
+        if (g_launch.p2g_waves == 4) {
+            if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1, 4>), G, dim3(256), 0, st, A);
+            else hipLaunchKernelGGL((k_p2g<T, 2, 4>), G, dim3(256), 0, st, A);
+            return;
+        }
+    }
+    {
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1, 2>), G, dim3(128), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g<T, 2, 2>), G, dim3(128), 0, st, A);
+    }
 }
 
 template <typename T>
@@ -1927,12 +1938,13 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
 }
 
 int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
-    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256};
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2};
     if (cfg) {
         if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
         if (cfg->block_wgs) c.block_wgs = cfg->block_wgs;
         if (cfg->p2g_pair) c.p2g_pair = cfg->p2g_pair;
         if (cfg->const_threads) c.const_threads = cfg->const_threads;
+        if (cfg->p2g_waves) c.p2g_waves = cfg->p2g_waves;
     }
     if (c.bucket_wgs < 8 || c.bucket_wgs > 16384 || c.bucket_wgs % 8 != 0)
         return fail(PMPM_EARG, "%sbucket_wgs must be a multiple of 8 in [8, 16384]: %lld", "", c.bucket_wgs);
@@ -1941,6 +1953,7 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
     if (c.p2g_pair != 1 && c.p2g_pair != 2) return fail(PMPM_EARG, "%sp2g_pair must be 1 or 2: %lld", "", c.p2g_pair);
     if (c.const_threads != 64 && c.const_threads != 128 && c.const_threads != 256)
         return fail(PMPM_EARG, "%sconst_threads must be 64, 128 or 256: %lld", "", c.const_threads);
+    if (c.p2g_waves != 2 && c.p2g_waves != 4) return fail(PMPM_EARG, "%sp2g_waves must be 2 or 4: %lld", "", c.p2g_waves);
     g_launch = c;
     return PMPM_OK;
 }

@@ -31,15 +31,19 @@ def test_work_group_counts_leave_the_step_bit_identical():
     ref = run(None)
     for cfg in ({"bucket_wgs": 1024, "block_wgs": 512}, {"bucket_wgs": 4096, "const_threads": 64},
                 {"block_wgs": 1, "bucket_wgs": 8, "const_threads": 128}):
+        # (the number of P2G waves changes how a bucket's particles are split
+        # among per-wave arenas, hence the summation order: compared below)
         sim = run(cfg)
         for k in STATE:
             assert torch.equal(getattr(sim, k), getattr(ref, k)), (cfg, k, rel_err(getattr(sim, k), getattr(ref, k)))
 
 
 def test_single_particle_p2g_matches_paired_to_rounding():
-    a, b = run({"p2g_pair": 2}), run({"p2g_pair": 1})
-    for k in ("x", "v", "F", "C"):
-        assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, k
+    a = run({"p2g_pair": 2})
+    for cfg in ({"p2g_pair": 1}, {"p2g_waves": 4}, {"p2g_waves": 4, "p2g_pair": 1}):
+        b = run(cfg)
+        for k in ("x", "v", "F", "C"):
+            assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, (cfg, k)
 
 
 def test_pcg_reduction_shape_changes_only_rounding():
@@ -56,7 +60,8 @@ def test_tune_solver_restores_state_and_picks_a_valid_config():
     sim.substep(0.0)
     before = {k: getattr(sim, k).clone() for k in STATE}
     step0 = sim._step
-    small = {"bucket_wgs": (1024, 2048), "p2g_pair": (1, 2), "const_threads": (128, 256), "block_wgs": (1024, 2048)}
+    small = {"bucket_wgs": (1024, 2048), "p2g_pair": (1, 2), "const_threads": (128, 256), "block_wgs": (1024, 2048),
+             "p2g_waves": (2, 4)}
     best = autotune.tune_solver(sim, steps=2, space=small, use_cache=False)
     assert autotune.get_config() == best
     assert sim._step == step0
