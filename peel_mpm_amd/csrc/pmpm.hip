@@ -6,8 +6,9 @@
 //   k_scan_tiles / k_scan_apply
 //                 bucket offsets (exclusive scan), ordered list of non-empty
 //                 buckets, marks on the grid blocks their stencils reach
-//   k_bin_scatter perm[start[key]+slot] = particle   (+ k_bin_stabilize)
-//   k_mark_compact list of marked grid blocks (clears the marks); k_inv_perm
+//   k_bin_scatter perm[start[key]+slot] = particle and its inverse inv
+//                 (+ k_bin_stabilize: ascending particle ids per cell)
+//   k_mark_compact list of marked grid blocks (clears the marks)
 //   k_const       per particle, in particle order: stress/damage/degradation
 //                 in registers, then the particle's P2G record (x, affine
 //                 momentum, mass*v, mass) and d at its sorted position
@@ -356,15 +357,21 @@ __global__ void __launch_bounds__(WG) k_mark_compact(int* __restrict__ mark, lon
 }
 
 __global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __restrict__ key, const int* __restrict__ slot,
-                                                    const int* __restrict__ start, int* __restrict__ perm) {
+                                                    const int* __restrict__ start, int* __restrict__ perm,
+                                                    int* __restrict__ inv) {
     long long p = (long long)blockIdx.x * WG + threadIdx.x;
-    if (p < n) perm[start[key[p]] + slot[p]] = (int)p;
+    if (p < n) {
+        const int pos = start[key[p]] + slot[p];
+        perm[pos] = (int)p;
+        inv[p] = pos;
+    }
 }
 
 // Sort each cell's particle ids ascending (then perm is a stable argsort of
 // key): one lane per cell of each non-empty bucket, insertion sort in place.
 __global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ start, const int* __restrict__ alist,
-                                                      const int* __restrict__ misc, int* __restrict__ perm) {
+                                                      const int* __restrict__ misc, int* __restrict__ perm,
+                                                      int* __restrict__ inv) {
     const int nact = misc[0];
     for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
         const long long c = ((long long)alist[ai] << 6) + threadIdx.x;
@@ -386,7 +393,10 @@ __global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ st
                 }
 #pragma unroll
             for (int i = 0; i < 16; ++i)
-                if (i < n) perm[s0 + i] = v[i];
+                if (i < n) {
+                    perm[s0 + i] = v[i];
+                    inv[v[i]] = s0 + i;
+                }
             continue;
         }
         for (int i = s0 + 1; i < s1; ++i) {
@@ -398,6 +408,7 @@ __global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ st
             }
             perm[j + 1] = e;
         }
+        for (int i = s0; i < s1; ++i) inv[perm[i]] = i;
     }
 }
 
@@ -640,12 +651,6 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
             for (int j = 0; j < 3; ++j) tau.m[i][j] = tpar.m[i][j] + gI * tperp.m[i][j] + g * (tpq.m[i][j] + tqp.m[i][j]);
     }
     return tau;
-}
-
-// Inverse of the sort permutation: inv[perm[i]] = i.
-__global__ void __launch_bounds__(WG) k_inv_perm(long long n, const int* __restrict__ perm, int* __restrict__ inv) {
-    const long long i = (long long)blockIdx.x * WG + threadIdx.x;
-    if (i < n) inv[perm[i]] = (int)i;
 }
 
 // Stress, damage and degradation per particle, in particle order (the
@@ -1648,6 +1653,7 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     int* key = (int*)(ws + L.key);
     int* slot = (int*)(ws + L.slot);
     int* perm = (int*)(ws + L.perm);
+    int* inv = (int*)(ws + L.inv);
     int* misc = (int*)(ws + L.misc);
     TimedScope ts(0, st);
     if (n > 0)
@@ -1657,9 +1663,9 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, L.ntiles);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, nb, tiles, L.ntiles, start, alist,
                        slotof, mark, misc);
-    if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm);
+    if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm, inv);
     if (stable && n > 0)
-        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm);
+        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm, inv);
     HIPCHK(hipGetLastError());
     return PMPM_OK;
 }
@@ -1763,9 +1769,6 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         TimedScope ts(0, st);
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
-        if (in->n > 0)
-            hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n,
-                               (const int*)(ws + L.perm), (int*)(ws + L.inv));
     }
     if (p->export_active)
         hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
@@ -1855,11 +1858,7 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
         if (rc) return rc;
         hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
                            (int*)(ws + L.blist), (int*)(ws + L.misc));
-        if (in->n > 0) {
-            hipLaunchKernelGGL(k_inv_perm, dim3(nblocks(in->n)), dim3(WG), 0, st, (long long)in->n,
-                               (const int*)(ws + L.perm), (int*)(ws + L.inv));
-            launch_transfer<T>(A, p->do_damage != 0, false, st);
-        }
+        if (in->n > 0) launch_transfer<T>(A, p->do_damage != 0, false, st);
         hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     }
     AdjArgs<T> B;
