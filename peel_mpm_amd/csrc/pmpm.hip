@@ -736,11 +736,22 @@ __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int 
 // contributions are summed in registers, then one read-modify-write of the
 // node's (m, mv) and (sum w d, sum w).  Lanes own distinct cells, so for a
 // fixed node offset the 64 lanes touch 64 distinct entries and no atomics
-// are needed; a compiler barrier after each node keeps the accesses of
-// different lanes to one entry in program order (LDS executes one
-// wavefront's accesses in order).
+// are needed.  Different nodes of different lanes can be one entry, so the
+// accesses must reach the LDS in program order, which executes one
+// wavefront's accesses in order: a compiler barrier follows every write.
+// Software pipeline: the next node's entry is read right after this node's
+// write (so after it, in LDS order), and the next node's contributions are
+// computed while that read is in flight; a scheduling barrier keeps that
+// arithmetic behind the read.
 template <typename T, int NP>
 __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp[3][4], const P2GPart<T> (&q)[NP]) {
+    T c0, c1, c2, c3, f0, f1;
+    {
+        const int l = (lp[0][0] * AR + lp[1][0]) * AR + lp[2][0];
+        c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3];
+        f0 = sc[l][0]; f1 = sc[l][1];
+    }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ox = 0; ox < 4; ++ox) {
         T bx[NP][3];
@@ -761,28 +772,33 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
             const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
-                T a0 = T(0), a1 = T(0), a2 = T(0), a3 = T(0), e0 = T(0), e1 = T(0);
+                T a0, a1, a2, a3, e0, e1;
 #pragma unroll
                 for (int j = 0; j < NP; ++j) {
                     const T wk = wqxy[j] * q[j].wq[oz][2];
                     const T wck = wcxy[j] * q[j].wc[oz][2];
-                    a0 += wk * q[j].mass;
-                    a1 += wk * (bxy[j][0] + q[j].af[0][2] * q[j].dq[oz][2]);
-                    a2 += wk * (bxy[j][1] + q[j].af[1][2] * q[j].dq[oz][2]);
-                    a3 += wk * (bxy[j][2] + q[j].af[2][2] * q[j].dq[oz][2]);
-                    e0 += wck * q[j].d;
-                    e1 += wck;
+                    const T t0 = wk * q[j].mass;
+                    const T t1 = wk * (bxy[j][0] + q[j].af[0][2] * q[j].dq[oz][2]);
+                    const T t2 = wk * (bxy[j][1] + q[j].af[1][2] * q[j].dq[oz][2]);
+                    const T t3 = wk * (bxy[j][2] + q[j].af[2][2] * q[j].dq[oz][2]);
+                    const T t4 = wck * q[j].d;
+                    if (j == 0) { a0 = t0; a1 = t1; a2 = t2; a3 = t3; e0 = t4; e1 = wck; }
+                    else { a0 += t0; a1 += t1; a2 += t2; a3 += t3; e0 += t4; e1 += wck; }
                 }
                 const int l = lxy + lp[2][oz];
-                T* g = sq[l];
-                T c0 = g[0], c1 = g[1], c2 = g[2], c3 = g[3];
                 c0 += a0; c1 += a1; c2 += a2; c3 += a3;
-                g[0] = c0; g[1] = c1; g[2] = c2; g[3] = c3;
-                T* cc = sc[l];
-                T f0 = cc[0], f1 = cc[1];
+                sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3;
                 f0 += e0; f1 += e1;
-                cc[0] = f0; cc[1] = f1;
+                sc[l][0] = f0; sc[l][1] = f1;
                 __asm__ __volatile__("" ::: "memory");
+                const int n = (ox * 4 + oy) * 4 + oz + 1;
+                if (n < 64) {
+                    const int nx = n >> 4, ny = (n >> 2) & 3, nz = n & 3;
+                    const int ln = (lp[0][nx] * AR + lp[1][ny]) * AR + lp[2][nz];
+                    c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3];
+                    f0 = sc[ln][0]; f1 = sc[ln][1];
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -798,7 +814,7 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 // stores.
 constexpr int P2G_WG = 256;
 template <typename T, int NP, int P2G_WAVES>
-__global__ void __launch_bounds__(64 * P2G_WAVES) k_p2g(StepArgs<T> A) {
+__global__ void __launch_bounds__(64 * P2G_WAVES) __attribute__((amdgpu_waves_per_eu(3))) k_p2g(StepArgs<T> A) {
     constexpr int NT = 64 * P2G_WAVES;
     // per wave: node-major (m, mvx, mvy, mvz) and (sum w d, sum w), so a node's
     // update is one 16-byte and one 8-byte LDS access (fp32)
