@@ -145,7 +145,8 @@ struct Layout {
         inv = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);              // sorted position of each particle
         rec = o;    o = al(o + (size_t)(n > 0 ? n : 1) * P2G_REC * ts);   // P2G records, sorted order
         dsrt = o;   o = al(o + (size_t)(n > 0 ? n : 1) * ts);             // post-update d, sorted order
-        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks
+        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2] grid partials,
+                                             // [3] some stencil leaves the grid (this step), [4] its raw flag
         stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
         gmw = o;    o = al(o + (size_t)G * 2 * ts);                // per node (m, sum w) kept for the adjoint
@@ -162,11 +163,15 @@ struct Layout {
 // clamped to the grid's buckets.  A bucket's particles are contiguous and
 // grouped by cell, which lets P2G put one cell on each lane.
 template <typename T>
-__device__ __forceinline__ int cell_key(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb) {
+// off_grid: the particle's cubic stencil [c-1, c+2] (which holds its
+// quadratic one) reaches past the grid on some axis.
+__device__ __forceinline__ int cell_key(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb, bool& off_grid) {
     int b[3], l[3];
+    off_grid = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const int c = floor_idx(scaled(x[3 * p + a], inv_dx), ng);
+        off_grid = off_grid || c < 1 || c + 2 > ng - 1;
         b[a] = clampi(c >> 2, 0, nb - 1);
         l[a] = clampi(c - BLK * b[a], 0, BLK - 1);
     }
@@ -184,19 +189,23 @@ __device__ __forceinline__ int scene_of(const long long* __restrict__ off, int n
 }
 
 // Slot within the cell: one atomic per run of equal keys in the wavefront.
+// misc[4] is raised (plain store) when a particle's stencil reaches past the
+// grid; k_scan_top publishes it as misc[3] for the step.
 template <typename T>
 __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long long n, T inv_dx, int ng, int nb,
                                                   const long long* __restrict__ scene_off, int ns,
                                                   int* __restrict__ count, int* __restrict__ key,
-                                                  int* __restrict__ slot) {
+                                                  int* __restrict__ slot, int* __restrict__ misc) {
     long long p = (long long)blockIdx.x * WG + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool live = p < n;
     int k = -1;
+    bool off_grid = false;
     if (live) {
-        k = cell_key(x, p, inv_dx, ng, nb);
+        k = cell_key(x, p, inv_dx, ng, nb, off_grid);
         if (ns > 1) k += scene_of(scene_off, ns, p) * (nb * nb * nb * 64);
     }
+    if (__ballot(off_grid) != 0 && lane == 0) misc[4] = 1;
     int kprev = __shfl_up(k, 1);
     bool head = live && (lane == 0 || kprev != k);
     unsigned long long heads = __ballot(head);
@@ -261,9 +270,15 @@ __device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) 
 
 // Exclusive scan of the per-tile (particles, non-empty buckets) totals, in
 // place; one work-group of 1024 threads.
-__global__ void __launch_bounds__(1024) k_scan_top(int* __restrict__ tiles, long long ntiles) {
+// Also publishes k_bin_count's off-grid flag for this step (misc[3] =
+// misc[4]) and clears the raw flag for the next binning.
+__global__ void __launch_bounds__(1024) k_scan_top(int* __restrict__ tiles, long long ntiles, int* __restrict__ misc) {
     __shared__ int s_a[16], s_b[16], s_ca, s_cb;
-    if (threadIdx.x == 0) { s_ca = 0; s_cb = 0; }
+    if (threadIdx.x == 0) {
+        s_ca = 0; s_cb = 0;
+        misc[3] = misc[4];
+        misc[4] = 0;
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (long long t0 = 0; t0 < ntiles; t0 += 1024) {
@@ -901,81 +916,109 @@ __global__ void __launch_bounds__(64 * P2G_WAVES) __attribute__((amdgpu_waves_pe
 }
 
 // ------------------------------------------------------------------ arena fold
-// Arena slots of the 27 buckets around block (bx,by,bz) (-1: none), fetched
-// together (uniform loads).
+// Arena slot of bucket k = lane of the 27 around block (bx,by,bz), k = 9 dx +
+// 3 dy + dz for bucket offsets dx, dy, dz + 1 in {0,1,2} (-1: outside the
+// grid or empty), on lanes 0..26; -1 on the other lanes.
 template <typename T>
-__device__ __forceinline__ void bucket_slots(const StepArgs<T>& A, int scene, int bx, int by, int bz, int slot[27]) {
+__device__ __forceinline__ int bucket_slot_lane(const StepArgs<T>& A, int scene, int bx, int by, int bz, int lane) {
     const int nb = A.nb;
-    const int* sl = A.slotof + scene * A.NBs;
-#pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const int u = bx - 1 + k / 9, v = by - 1 + (k / 3) % 3, w = bz - 1 + k % 3;
-        const bool in = u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
-        slot[k] = in ? sl[(u * nb + v) * nb + w] : -1;
-    }
+    const int u = bx - 1 + lane / 9, v = by - 1 + (lane / 3) % 3, w = bz - 1 + lane % 3;
+    const bool in = lane < 27 && u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
+    return in ? A.slotof[scene * A.NBs + (u * nb + v) * nb + w] : -1;
 }
 
-// acc[c] += the node's channel c, c < NCH, from its 6 node-major values
-// (8-byte pairs in fp32: a node's values start 24-byte aligned).
-template <typename T, int NCH> __device__ __forceinline__ void add_node(const T* __restrict__ src, T acc[NCH]) {
+// The first NCH channels of a node's 6 node-major values (8-byte pairs in
+// fp32: a node's values start 24-byte aligned).
+template <typename T, int NCH> __device__ __forceinline__ void load_node(const T* __restrict__ src, T out[NCH]) {
     if constexpr (sizeof(T) == 4 && NCH % 2 == 0) {
         const float2* s2 = reinterpret_cast<const float2*>(src);
 #pragma unroll
         for (int c = 0; c < NCH / 2; ++c) {
             const float2 v = s2[c];
-            acc[2 * c] += v.x;
-            acc[2 * c + 1] += v.y;
+            out[2 * c] = v.x;
+            out[2 * c + 1] = v.y;
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) acc[c] += src[c];
+        for (int c = 0; c < NCH; ++c) out[c] = src[c];
     }
 }
 
-// Sum of the first NCH channels of node g over the arenas that hold it (the
-// nodes of block b lie in the arenas [4b'-1, 4b'+5] of buckets b' = b-1..b+1
-// per axis), in a fixed bucket order; edge nodes 0 and ng-1 also collect the
-// arena positions beyond the grid (the frames keep those positions
-// unclamped; the reference clamps such nodes onto the edge node).
+// Sum of the first NCH channels of node g over the arenas that hold it, in
+// a fixed bucket order (ascending bucket offsets).  Arena [4b'-1, 4b'+5]
+// holds node 4b+i of block b for b' = b-1 (i = 0, 1), b' = b (all i) and
+// b' = b+1 (i = 3): one or two buckets per axis, at most 8 in all.  The 8
+// candidates are loaded unconditionally -- a missing one reads arena slot 0
+// and is masked to zero -- so all loads are in flight together.  lslot:
+// this wavefront's bucket_slot_lane (every lane of the wavefront must be
+// here).  Edge nodes 0 and ng-1 also collect the arena positions beyond
+// the grid (the frames keep those positions unclamped; the reference clamps
+// such nodes onto the edge node); off_grid: some particle's stencil reached
+// past the grid this step -- otherwise those positions hold zeros.
 template <typename T, int NCH>
-__device__ __forceinline__ void fold_node(const T* __restrict__ arena, const int slot[27], const int g[3], int bx, int by,
-                                          int bz, int ng, T acc[NCH]) {
-    // each node's own position in every arena that holds it: all loads are
-    // independent and issued together
+__device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int lslot, const int g[3], int bx, int by, int bz,
+                                          bool off_grid, T acc[NCH]) {
+    const T* __restrict__ arena = A.arena;
+    const int ng = A.ng;
+    const int b[3] = {bx, by, bz};
+    int lo[3];
+    bool two[3];
 #pragma unroll
-    for (int k = 0; k < 27; ++k) {
-        const int lx = g[0] - (BLK * (bx - 1 + k / 9) - 1);
-        const int ly = g[1] - (BLK * (by - 1 + (k / 3) % 3) - 1);
-        const int lz = g[2] - (BLK * (bz - 1 + k % 3) - 1);
-        if (slot[k] >= 0 && lx >= 0 && lx < AR && ly >= 0 && ly < AR && lz >= 0 && lz < AR)
-            add_node<T, NCH>(arena + (size_t)slot[k] * (6 * AR3) + 6 * ((lx * AR + ly) * AR + lz), acc);
+    for (int a = 0; a < 3; ++a) {
+        const int i = g[a] - BLK * b[a];
+        lo[a] = i < 2 ? -1 : 0;
+        two[a] = i != 2;
     }
+    T v[8][NCH];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+        const int cx = c >> 2, cy = (c >> 1) & 1, cz = c & 1;
+        const int ux = lo[0] + cx, uy = lo[1] + cy, uz = lo[2] + cz;
+        const int s = __shfl(lslot, (ux + 1) * 9 + (uy + 1) * 3 + (uz + 1));
+        const bool ok = (cx == 0 || two[0]) && (cy == 0 || two[1]) && (cz == 0 || two[2]) && s >= 0;
+        const int lx = g[0] - (BLK * (bx + ux) - 1), ly = g[1] - (BLK * (by + uy) - 1), lz = g[2] - (BLK * (bz + uz) - 1);
+        const size_t off = ok ? (size_t)s * (6 * AR3) + 6 * ((lx * AR + ly) * AR + lz) : 0;
+        load_node<T, NCH>(arena + off, v[c]);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) v[c][ch] = ok ? v[c][ch] : T(0);
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) acc[ch] += v[c][ch];
     const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
-    if (!edge) return;
+    if (!off_grid || !edge) return;
+    const int nb = A.nb;
+    const int* sl = A.slotof + scene * A.NBs;
     for (int k = 0; k < 27; ++k) {
-        if (slot[k] < 0) continue;
         const int b3[3] = {bx - 1 + k / 9, by - 1 + (k / 3) % 3, bz - 1 + k % 3};
-        int lo[3], hi[3];
+        if (b3[0] < 0 || b3[0] >= nb || b3[1] < 0 || b3[1] >= nb || b3[2] < 0 || b3[2] >= nb) continue;
+        const int sk = sl[(b3[0] * nb + b3[1]) * nb + b3[2]];
+        if (sk < 0) continue;
+        int lo3[3], hi3[3];
         bool any_extra = false, ok = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int org = BLK * b3[a] - 1;
             const int plo = g[a] == 0 ? org : g[a];
             const int phi = g[a] == ng - 1 ? org + AR - 1 : g[a];
-            lo[a] = (plo > org ? plo : org) - org;
-            hi[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
-            ok = ok && lo[a] <= hi[a];
-            any_extra = any_extra || lo[a] != hi[a] || lo[a] != g[a] - org;
+            lo3[a] = (plo > org ? plo : org) - org;
+            hi3[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
+            ok = ok && lo3[a] <= hi3[a];
+            any_extra = any_extra || lo3[a] != hi3[a] || lo3[a] != g[a] - org;
         }
         if (!ok || !any_extra) continue;
-        const T* base = arena + (size_t)slot[k] * (6 * AR3);
-        for (int lx = lo[0]; lx <= hi[0]; ++lx)
-            for (int ly = lo[1]; ly <= hi[1]; ++ly)
-                for (int lz = lo[2]; lz <= hi[2]; ++lz) {
+        const T* base = arena + (size_t)sk * (6 * AR3);
+        for (int lx = lo3[0]; lx <= hi3[0]; ++lx)
+            for (int ly = lo3[1]; ly <= hi3[1]; ++ly)
+                for (int lz = lo3[2]; lz <= hi3[2]; ++lz) {
                     // the node's own position was summed above
                     if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) && lz == g[2] - (BLK * b3[2] - 1))
                         continue;
-                    add_node<T, NCH>(base + 6 * ((lx * AR + ly) * AR + lz), acc);
+                    T e[NCH];
+                    load_node<T, NCH>(base + 6 * ((lx * AR + ly) * AR + lz), e);
+#pragma unroll
+                    for (int ch = 0; ch < NCH; ++ch) acc[ch] += e[ch];
                 }
     }
 }
@@ -997,6 +1040,7 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
     const int nwaves = gridDim.x * (WG / 64);
     const int nblocks = A.misc[1];
+    const bool off_grid = A.misc[3] != 0;
     const int nb = A.nb, ng = A.ng;
     unsigned long long nq = 0, nc = 0;
     for (int i = wave; i < nblocks; i += nwaves) {
@@ -1004,11 +1048,9 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
         const int scene = (int)(blk / A.NBs), loc = (int)(blk - scene * A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
-        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        int slot[27];
-        bucket_slots(A, scene, bx, by, bz, slot);
-        fold_node<T, 6>(A.arena, slot, g, bx, by, bz, ng, acc);
+        fold_node<T, 6>(A, scene, bucket_slot_lane(A, scene, bx, by, bz, lane), g, bx, by, bz, off_grid, acc);
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         const int gx = g[0], gy = g[1], gz = g[2];
         const long long node = scene * A.Gs + ((long long)gx * ng + gy) * ng + gz;
         const T m = acc[0], mvx = acc[1], mvy = acc[2], mvz = acc[3], dsum = acc[4], wsum = acc[5];
@@ -1072,6 +1114,96 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     }
 }
 
+// G2P gathers of one particle from the bucket's staged nodes s_g (velocity
+// in channels 0-2, normalised damage in 3), as separable sums -- over z per
+// (x, y) row, then over y per x plane, then over x:
+//   v = sum w g,  C[a][b] = sum w g_a dp_b  (anisompm.py:751-760, before the
+//   4 inv_dx^2 factor),
+//   lap = sum (w''x wy wz + wx w''y wz + wx wy w''z) gd  (:781-791, before
+//   the inv_dx^2 factor).
+// base: the stencil's first node per axis; org: the arena's first node.  A
+// stencil node clamps to the grid as in the reference (anisompm.py:586-590).
+// UNCLAMPED: no node of the stencil needs the clamp, so every node is the
+// stencil base plus a compile-time offset.
+template <int W, bool UNCLAMPED>
+__device__ __forceinline__ void arena_pos(const int base[3], const int org[3], int ng, int& b0, int lp[3][W]) {
+    b0 = ((base[0] - org[0]) * AR + (base[1] - org[1])) * AR + (base[2] - org[2]);
+    if constexpr (!UNCLAMPED) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int o = 0; o < W; ++o) lp[a][o] = clampi(base[a] + o, 0, ng - 1) - org[a];
+    }
+}
+template <typename T, bool UNCLAMPED>
+__device__ __forceinline__ void g2p_gather_q(const T (*s_g)[AR3], const int bq[3], const int org[3], int ng, const T w[3][3],
+                                             const T fx[3], T dx, T vn[3], M3<T>& Cn) {
+    int q0, lq[3][3];
+    arena_pos<3, UNCLAMPED>(bq, org, ng, q0, lq);
+    T wd[3][3];                 // weight times node offset dp, per node and axis
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) wd[o][a] = w[o][a] * ((T(o) - fx[a]) * dx);
+#pragma unroll
+    for (int ox = 0; ox < 3; ++ox) {
+        T sy[3], syd[3], ty[3];
+#pragma unroll
+        for (int oy = 0; oy < 3; ++oy) {
+            T sz[3], tz[3];
+#pragma unroll
+            for (int oz = 0; oz < 3; ++oz) {
+                const int l = UNCLAMPED ? q0 + (ox * AR + oy) * AR + oz : (lq[0][ox] * AR + lq[1][oy]) * AR + lq[2][oz];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const T g = s_g[a][l];
+                    sz[a] = oz == 0 ? w[0][2] * g : sz[a] + w[oz][2] * g;
+                    tz[a] = oz == 0 ? wd[0][2] * g : tz[a] + wd[oz][2] * g;
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                sy[a] = oy == 0 ? w[0][1] * sz[a] : sy[a] + w[oy][1] * sz[a];
+                syd[a] = oy == 0 ? wd[0][1] * sz[a] : syd[a] + wd[oy][1] * sz[a];
+                ty[a] = oy == 0 ? w[0][1] * tz[a] : ty[a] + w[oy][1] * tz[a];
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vn[a] = ox == 0 ? w[0][0] * sy[a] : vn[a] + w[ox][0] * sy[a];
+            Cn.m[a][0] = ox == 0 ? wd[0][0] * sy[a] : Cn.m[a][0] + wd[ox][0] * sy[a];
+            Cn.m[a][1] = ox == 0 ? w[0][0] * syd[a] : Cn.m[a][1] + w[ox][0] * syd[a];
+            Cn.m[a][2] = ox == 0 ? w[0][0] * ty[a] : Cn.m[a][2] + w[ox][0] * ty[a];
+        }
+    }
+}
+template <typename T, bool UNCLAMPED>
+__device__ __forceinline__ void g2p_gather_c(const T (*s_g)[AR3], const int bc[3], const int org[3], int ng,
+                                             const T wc[4][3], const T w2[4][3], T& lap) {
+    int k0, lc[3][4];
+    arena_pos<4, UNCLAMPED>(bc, org, ng, k0, lc);
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox) {
+        T py, qy, ry;           // sum_y wy (sum_z wz gd), sum_y w''y (..), sum_y wy (sum_z w''z gd)
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            T az, bz;           // sum_z wz gd, sum_z w''z gd
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz) {
+                const int l = UNCLAMPED ? k0 + (ox * AR + oy) * AR + oz : (lc[0][ox] * AR + lc[1][oy]) * AR + lc[2][oz];
+                const T g = s_g[3][l];
+                az = oz == 0 ? wc[0][2] * g : az + wc[oz][2] * g;
+                bz = oz == 0 ? w2[0][2] * g : bz + w2[oz][2] * g;
+            }
+            py = oy == 0 ? wc[0][1] * az : py + wc[oy][1] * az;
+            qy = oy == 0 ? w2[0][1] * az : qy + w2[oy][1] * az;
+            ry = oy == 0 ? wc[0][1] * bz : ry + wc[oy][1] * bz;
+        }
+        const T px = w2[ox][0] * py + wc[ox][0] * (qy + ry);
+        lap = ox == 0 ? px : lap + px;
+    }
+}
+
 // G2P + cubic Laplacian + advection/F update/clamp per bucket (anisompm.py:748-803).
 template <typename T>
 __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
@@ -1103,59 +1235,31 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
             int bq[3];
             T fx[3], w[3][3];
             quad_weights(Xp, ng, bq, fx, w);
-            T vn[3] = {T(0), T(0), T(0)};
-            M3<T> Cn;
+            // a particle whose stencils need no grid clamp takes the fast
+            // path (the cubic base is floor(Xp) - 1, as in cubic_weights)
+            int bcb[3];
+            const int org[3] = {o0, o1, o2};
+            bool unclamped = true;
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int b = 0; b < 3; ++b) Cn.m[a][b] = T(0);
-#pragma unroll
-            for (int ox = 0; ox < 3; ++ox) {
-                const int lx = clampi(bq[0] + ox, 0, ng - 1) - o0;
-#pragma unroll
-                for (int oy = 0; oy < 3; ++oy) {
-                    const int ly = clampi(bq[1] + oy, 0, ng - 1) - o1;
-                    const T wxy = w[ox][0] * w[oy][1];
-#pragma unroll
-                    for (int oz = 0; oz < 3; ++oz) {
-                        const int lz = clampi(bq[2] + oz, 0, ng - 1) - o2;
-                        const int l = (lx * AR + ly) * AR + lz;
-                        const T wk = wxy * w[oz][2];
-                        const T g[3] = {s_g[0][l], s_g[1][l], s_g[2][l]};
-                        const T dp[3] = {(T(ox) - fx[0]) * A.dx, (T(oy) - fx[1]) * A.dx, (T(oz) - fx[2]) * A.dx};
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            vn[a] = vn[a] + wk * g[a];
-#pragma unroll
-                            for (int b = 0; b < 3; ++b) Cn.m[a][b] = Cn.m[a][b] + wk * (g[a] * dp[b]);
-                        }
-                    }
-                }
+            for (int a = 0; a < 3; ++a) {
+                bcb[a] = floor_idx(Xp[a], ng) - 1;
+                unclamped = unclamped && bq[a] >= 0 && bq[a] + 2 < ng && bcb[a] >= 0 && bcb[a] + 3 < ng;
             }
+            T vn[3], lap;
+            M3<T> Cn;
+            if (unclamped) g2p_gather_q<T, true>(s_g, bq, org, ng, w, fx, A.dx, vn, Cn);
+            else g2p_gather_q<T, false>(s_g, bq, org, ng, w, fx, A.dx, vn, Cn);
 #pragma unroll
             for (int a = 0; a < 3; ++a)
 #pragma unroll
                 for (int b = 0; b < 3; ++b) Cn.m[a][b] = Cn.m[a][b] * A.c_scale;
-            int bcb[3];
-            T wc[4][3], w2[4][3];
-            cubic_weights(Xp, ng, bcb, wc, w2);
-            T lap = T(0);
-#pragma unroll
-            for (int ox = 0; ox < 4; ++ox) {
-                const int lx = clampi(bcb[0] + ox, 0, ng - 1) - o0;
-#pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const int ly = clampi(bcb[1] + oy, 0, ng - 1) - o1;
-#pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const int lz = clampi(bcb[2] + oz, 0, ng - 1) - o2;
-                        const int l = (lx * AR + ly) * AR + lz;
-                        const T lw = A.lap_scale * (w2[ox][0] * wc[oy][1] * wc[oz][2] + wc[ox][0] * w2[oy][1] * wc[oz][2] +
-                                                    wc[ox][0] * wc[oy][1] * w2[oz][2]);
-                        lap = lap + lw * s_g[3][l];
-                    }
-                }
+            {
+                T wc[4][3], w2[4][3];
+                cubic_weights(Xp, ng, bcb, wc, w2);
+                if (unclamped) g2p_gather_c<T, true>(s_g, bcb, org, ng, wc, w2, lap);
+                else g2p_gather_c<T, false>(s_g, bcb, org, ng, wc, w2, lap);
             }
+            lap = A.lap_scale * lap;
             const M3<T> F = load_m3(A.use_fclamp ? (const T*)A.oF : A.F, p);
             M3<T> Ip;
 #pragma unroll
@@ -1404,6 +1508,7 @@ __global__ void __launch_bounds__(WG) k_pcg_fold(StepArgs<T> A) {
     const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
     const int nwaves = gridDim.x * (WG / 64);
     const int nblocks = A.misc[1];
+    const bool off_grid = A.misc[3] != 0;
     const int nb = A.nb, ng = A.ng;
     const T beta = T(A.pscal[PCG_BETA]);
     double part[1] = {0.0};
@@ -1411,13 +1516,11 @@ __global__ void __launch_bounds__(WG) k_pcg_fold(StepArgs<T> A) {
         const int blk = A.blist[i];
         const int bx = blk / (nb * nb), by = (blk / nb) % nb, bz = blk % nb;
         const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
-        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         T acc[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
-        int slot[27];
-        bucket_slots(A, 0, bx, by, bz, slot);
-        fold_node<T, NCH>(A.arena, slot, g, bx, by, bz, ng, acc);
+        fold_node<T, NCH>(A, 0, bucket_slot_lane(A, 0, bx, by, bz, lane), g, bx, by, bz, off_grid, acc);
+        if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         const long long node = ((long long)g[0] * ng + g[1]) * ng + g[2];
         T* v = A.gv + 4 * node;
         T* dg = A.pdiag + 2 * node;
@@ -1674,9 +1777,9 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     TimedScope ts(0, st);
     if (n > 0)
         hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, key,
-                           slot);
+                           slot, misc);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, tiles);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, L.ntiles);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, L.ntiles, misc);
     hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, nb, tiles, L.ntiles, start, alist,
                        slotof, mark, misc);
     if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm, inv);
