@@ -1,14 +1,17 @@
 // libpmpm: MI355X (gfx950) kernels of the AnisoMPM substep and their C ABI.
 //
 // One step (anisompm_py/src/anisompm.py:593-803) runs as:
-//   k_bin_count   key = 4x4x4 cell-block of each particle, per-bucket slots
-//                 (wave-aggregated atomics over runs of equal keys)
-//   k_scan_tiles / k_scan_apply
-//                 bucket offsets (exclusive scan), ordered list of non-empty
-//                 buckets, marks on the grid blocks their stencils reach
+//   k_bin_count   key = cell (bucket-major) of each particle, its slot in the
+//                 cell (wave-aggregated atomics over runs of equal keys),
+//                 flags on the touched 4x4x4-cell buckets
+//   k_bucket_tiles / k_bucket_apply
+//                 bucket totals of the flagged buckets, exclusive scan over
+//                 buckets: bucket and cell offsets, ordered list of
+//                 non-empty buckets, marks on the grid blocks their
+//                 stencils reach
+//   k_mark_compact list of marked grid blocks (clears the marks)
 //   k_bin_scatter perm[start[key]+slot] = particle and its inverse inv
 //                 (+ k_bin_stabilize: ascending particle ids per cell)
-//   k_mark_compact list of marked grid blocks (clears the marks)
 //   k_const       per particle, in particle order: stress/damage/degradation
 //                 in registers, then the particle's P2G record (x, affine
 //                 momentum, mass*v, mass) and d at its sorted position
@@ -43,8 +46,8 @@ constexpr int BLK = 4;                 // cells per bucket edge (= nodes per gri
 constexpr int AR = BLK + 3;            // arena edge: nodes [4b-1, 4b+5]
 constexpr int AR3 = AR * AR * AR;      // 343
 constexpr int WG = 256;                // threads per work-group
-constexpr int SCAN_TILE = 1024;        // buckets per scan tile (4 per thread)
 constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket kernels
+constexpr int BTILE = WG;              // buckets per bucket-scan tile
 constexpr int P2G_GRID = 2048;         // work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
@@ -108,14 +111,15 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
 pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2};
 
-// Workspace layout (byte offsets).  The step keeps count and mark at zero
-// between steps: whoever reads them clears them.  count/start are per cell
-// (bucket-major), alist/slotof/mark/blist per bucket or grid block.  arena
-// holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in
-// active-list order.
+// Workspace layout (byte offsets).  The step keeps count, bflag and mark at
+// zero between steps: whoever reads them clears them.  count/start are per
+// cell (bucket-major; start is kept for the cells of non-empty buckets),
+// btot/bstart/bflag/alist/slotof/mark/blist per bucket or grid block.  arena holds one
+// 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in active-list
+// order.
 struct Layout {
-    size_t arena, gv, active, count, start, tiles, alist, slotof, mark, blist, key, slot, perm, inv, rec, dsrt, misc, stats,
-        pcgd, pcgr, gmw, total;
+    size_t arena, gv, active, count, start, btot, bstart, bflag, tiles, mark, alist, slotof, blist, key, slot, perm, inv, rec,
+        dsrt, misc, stats, pcgd, pcgr, gmw, total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -126,7 +130,7 @@ struct Layout {
         NBs = (long long)nblk(ng) * nblk(ng) * nblk(ng);
         NB = NBs * ns;
         NC = NB * 64;                            // cells, bucket-major
-        ntiles = (NC + SCAN_TILE - 1) / SCAN_TILE;
+        ntiles = (NB + BTILE - 1) / BTILE;       // bucket-scan tiles
         max_slots = NB < (long long)(n > 0 ? n : 1) ? NB : (long long)(n > 0 ? n : 1);
         size_t o = 0;
         arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
@@ -134,10 +138,13 @@ struct Layout {
         active = o; o = al(o + (size_t)G);
         count = o;  o = al(o + (size_t)NC * 4);
         start = o;  o = al(o + (size_t)(NC + 1) * 4);
+        btot = o;   o = al(o + (size_t)NB * 4);
+        bstart = o; o = al(o + (size_t)NB * 4);
+        bflag = o;  o = al(o + (size_t)NB * 4);
         tiles = o;  o = al(o + (size_t)ntiles * 2 * 4);
+        mark = o;   o = al(o + (size_t)NB * 4);
         alist = o;  o = al(o + (size_t)NB * 4);
         slotof = o; o = al(o + (size_t)NB * 4);
-        mark = o;   o = al(o + (size_t)NB * 4);
         blist = o;  o = al(o + (size_t)NB * 4);
         key = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);
         slot = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
@@ -188,13 +195,31 @@ __device__ __forceinline__ int scene_of(const long long* __restrict__ off, int n
     return lo;
 }
 
-// Slot within the cell: one atomic per run of equal keys in the wavefront.
-// misc[4] is raised (plain store) when a particle's stencil reaches past the
-// grid; k_scan_top publishes it as misc[3] for the step.
+// Run of equal values v among the live lanes [0, nlive) of a wavefront:
+// whether this lane heads its run, the run's first lane and its length.
+__device__ __forceinline__ void wave_run(int v, bool live, int lane, bool& head, int& first, int& len) {
+    const int vprev = __shfl_up(v, 1);
+    head = live && (lane == 0 || vprev != v);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long alive = __ballot(live);
+    const unsigned long long upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
+    first = 63 - __builtin_clzll((heads & upto) | 1ull);     // highest head <= lane
+    const unsigned long long above = heads & ~upto;
+    int end = above ? __builtin_ctzll(above) : 64;
+    const int nlive = 64 - __builtin_clzll(alive | 1ull);   // live lanes are [0, nlive)
+    if (end > nlive) end = nlive;
+    len = end - first;
+}
+
+// Cell key and slot within the cell of each particle (one atomic per run
+// of equal keys in the wavefront); each touched bucket is flagged in bflag
+// (plain stores).  misc[4] is raised (plain store) when a particle's
+// stencil reaches past the grid; k_bucket_apply publishes it as misc[3]
+// for the step.
 template <typename T>
 __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long long n, T inv_dx, int ng, int nb,
                                                   const long long* __restrict__ scene_off, int ns,
-                                                  int* __restrict__ count, int* __restrict__ key,
+                                                  int* __restrict__ count, int* __restrict__ bflag, int* __restrict__ key,
                                                   int* __restrict__ slot, int* __restrict__ misc) {
     long long p = (long long)blockIdx.x * WG + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -206,139 +231,114 @@ __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long 
         if (ns > 1) k += scene_of(scene_off, ns, p) * (nb * nb * nb * 64);
     }
     if (__ballot(off_grid) != 0 && lane == 0) misc[4] = 1;
-    int kprev = __shfl_up(k, 1);
-    bool head = live && (lane == 0 || kprev != k);
-    unsigned long long heads = __ballot(head);
-    unsigned long long alive = __ballot(live);
-    // run of this lane: [hl, he)
-    unsigned long long upto = (lane == 63) ? ~0ull : ((1ull << (lane + 1)) - 1);
-    int hl = 63 - __builtin_clzll((heads & upto) | 1ull);     // highest head <= lane
-    unsigned long long above = heads & ~upto;
-    int he = above ? __builtin_ctzll(above) : 64;
-    int nalive_to = 64 - __builtin_clzll(alive | 1ull);      // live lanes are [0, nalive_to)
-    if (he > nalive_to) he = nalive_to;
+    bool head;
+    int first, len;
+    wave_run(k, live, lane, head, first, len);
     int base = 0;
-    if (head) base = atomicAdd(&count[k], he - hl);
-    base = __shfl(base, hl);
+    if (head) base = atomicAdd(&count[k], len);
+    base = __shfl(base, first);
     if (live) {
         key[p] = k;
-        slot[p] = base + (lane - hl);
+        slot[p] = base + (lane - first);
     }
+    const int kb = k >> 6, kbprev = __shfl_up(kb, 1);
+    if (live && (lane == 0 || kbprev != kb)) bflag[kb] = 1;
 }
 
-// A scan tile is 1024 cells = 16 buckets; each thread holds 4 cells and 16
-// threads hold one bucket.  Per tile: particle total, non-empty buckets.
-__global__ void __launch_bounds__(WG) k_scan_tiles(const int* __restrict__ count, long long NC, int* __restrict__ tiles) {
-    __shared__ int s_c[WG / 64], s_n[WG / 64];
-    const long long c0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
-    int c = 0;
+// Bucket scan, pass 1: one thread per bucket, a tile of BTILE buckets per
+// work-group.  A flagged bucket sums its 64 cell counts (btot); per tile:
+// particles and non-empty buckets.
+__global__ void __launch_bounds__(WG) k_bucket_tiles(const int* __restrict__ bflag, const int* __restrict__ count,
+                                                     long long NB, int* __restrict__ btot, int* __restrict__ tiles) {
+    __shared__ int s_a[WG / 64], s_b[WG / 64];
+    const long long b = (long long)blockIdx.x * BTILE + threadIdx.x;
+    int t = 0;
+    if (b < NB && bflag[b]) {
+        const int4* c4 = reinterpret_cast<const int4*>(count + (b << 6));
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-        if (c0 + i < NC) c += count[c0 + i];
-    int bsum = c;                                  // bucket total over 16 lanes
-    for (int o = 8; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 16);
-    int ne = ((threadIdx.x & 15) == 0 && bsum > 0) ? 1 : 0;
-    for (int o = 32; o > 0; o >>= 1) {
-        c += __shfl_xor(c, o);
-        ne += __shfl_xor(ne, o);
+        for (int i = 0; i < 16; ++i) {
+            const int4 v = c4[i];
+            t += v.x + v.y + v.z + v.w;
+        }
     }
-    if ((threadIdx.x & 63) == 0) { s_c[threadIdx.x >> 6] = c; s_n[threadIdx.x >> 6] = ne; }
+    if (b < NB) btot[b] = t;
+    int a = t, c = t > 0 ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        c += __shfl_xor(c, o);
+    }
+    if ((threadIdx.x & 63) == 0) { s_a[threadIdx.x >> 6] = a; s_b[threadIdx.x >> 6] = c; }
     __syncthreads();
     if (threadIdx.x == 0) {
-        int tc = 0, tn = 0;
-        for (int w = 0; w < WG / 64; ++w) { tc += s_c[w]; tn += s_n[w]; }
-        tiles[2 * blockIdx.x] = tc;
-        tiles[2 * blockIdx.x + 1] = tn;
+        int ta = 0, tc = 0;
+        for (int w = 0; w < WG / 64; ++w) { ta += s_a[w]; tc += s_b[w]; }
+        tiles[2 * blockIdx.x] = ta;
+        tiles[2 * blockIdx.x + 1] = tc;
     }
 }
 
-// Exclusive block scan of (a, b) pairs, one pair per thread.
-__device__ __forceinline__ void block_scan2(int& a, int& b, int* s_a, int* s_b) {
+// Bucket scan, pass 2 (same tiling): each work-group sums the totals of the
+// tiles before it, then scans its own buckets -- bucket starts (bstart,
+// every bucket), the ordered list of non-empty buckets (alist) and their
+// slots (slotof, -1 when empty), the cell starts of non-empty buckets (and
+// the end of each one's last cell), marks on the grid blocks their arenas
+// reach.  Clears count and bflag of the buckets it read.  Work-group 0 also
+// publishes k_bin_count's off-grid flag (misc[3] = misc[4], clearing
+// misc[4]) and resets misc[1]; the last writes misc[0] = non-empty buckets
+// and start[NC] = particles.
+__global__ void __launch_bounds__(WG) k_bucket_apply(int* __restrict__ bflag, int* __restrict__ count, const int* __restrict__ btot,
+                                                     long long NB, int nb, const int* __restrict__ tiles, int* __restrict__ bstart,
+                                                     int* __restrict__ alist, int* __restrict__ slotof, int* __restrict__ mark,
+                                                     int* __restrict__ start, int* __restrict__ misc) {
+    __shared__ int s_a[WG / 64], s_b[WG / 64], s_pa, s_pb;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int ia = a, ib = b;
+    // prefix of the earlier tiles (wave 0)
+    if (w == 0) {
+        int pa = 0, pb = 0;
+        for (int t = lane; t < (int)blockIdx.x; t += 64) { pa += tiles[2 * t]; pb += tiles[2 * t + 1]; }
+        for (int o = 32; o > 0; o >>= 1) {
+            pa += __shfl_xor(pa, o);
+            pb += __shfl_xor(pb, o);
+        }
+        if (lane == 0) { s_pa = pa; s_pb = pb; }
+    }
+    const long long b = (long long)blockIdx.x * BTILE + threadIdx.x;
+    const int t = b < NB ? btot[b] : 0;
+    int ia = t, ib = t > 0 ? 1 : 0;
     for (int o = 1; o < 64; o <<= 1) {
-        int ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
+        const int ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
         if (lane >= o) { ia += ta; ib += tb; }
     }
     if (lane == 63) { s_a[w] = ia; s_b[w] = ib; }
     __syncthreads();
-    int pa = 0, pb = 0;
+    int pa = s_pa, pb = s_pb;
     for (int j = 0; j < w; ++j) { pa += s_a[j]; pb += s_b[j]; }
-    a = pa + ia - a;
-    b = pb + ib - b;
-}
-
-// Exclusive scan of the per-tile (particles, non-empty buckets) totals, in
-// place; one work-group of 1024 threads.
-// Also publishes k_bin_count's off-grid flag for this step (misc[3] =
-// misc[4]) and clears the raw flag for the next binning.
-__global__ void __launch_bounds__(1024) k_scan_top(int* __restrict__ tiles, long long ntiles, int* __restrict__ misc) {
-    __shared__ int s_a[16], s_b[16], s_ca, s_cb;
-    if (threadIdx.x == 0) {
-        s_ca = 0; s_cb = 0;
-        misc[3] = misc[4];
-        misc[4] = 0;
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (long long t0 = 0; t0 < ntiles; t0 += 1024) {
-        const long long t = t0 + threadIdx.x;
-        const int a = t < ntiles ? tiles[2 * t] : 0, b = t < ntiles ? tiles[2 * t + 1] : 0;
-        int ia = a, ib = b;
-        for (int o = 1; o < 64; o <<= 1) {
-            int ta = __shfl_up(ia, o), tb = __shfl_up(ib, o);
-            if (lane >= o) { ia += ta; ib += tb; }
-        }
-        if (lane == 63) { s_a[w] = ia; s_b[w] = ib; }
-        __syncthreads();
-        int pa = s_ca, pb = s_cb;
-        for (int j = 0; j < w; ++j) { pa += s_a[j]; pb += s_b[j]; }
-        if (t < ntiles) { tiles[2 * t] = pa + ia - a; tiles[2 * t + 1] = pb + ib - b; }
-        __syncthreads();
-        if (threadIdx.x == 1023) { s_ca = pa + ia; s_cb = pb + ib; }
-        __syncthreads();
-    }
-}
-
-// Cell starts, the ordered list of non-empty buckets and their slots, marks
-// on the grid blocks their arenas reach; clears count.  tiles holds the
-// exclusive prefix of earlier tiles (k_scan_top).
-__global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long long NC, int nb, const int* __restrict__ tiles,
-                                                   long long ntiles, int* __restrict__ start, int* __restrict__ alist,
-                                                   int* __restrict__ slotof, int* __restrict__ mark, int* __restrict__ misc) {
-    __shared__ int s_a[WG / 64], s_b[WG / 64];
-    const int tpa = tiles[2 * blockIdx.x], tpb = tiles[2 * blockIdx.x + 1];
-    const long long c0 = (long long)blockIdx.x * SCAN_TILE + threadIdx.x * 4;
-    int v[4], a = 0;
+    pa += ia - t;                       // this bucket's first particle
+    pb += ib - (t > 0 ? 1 : 0);         // its slot if non-empty
+    if (b < NB) {
+        bstart[b] = pa;
+        slotof[b] = t > 0 ? pb : -1;
+        if (bflag[b]) bflag[b] = 0;
+        if (t > 0) {
+            alist[pb] = (int)b;
+            int4* c4 = reinterpret_cast<int4*>(count + (b << 6));
+            int4* s4 = reinterpret_cast<int4*>(start + (b << 6));
+            int a = pa;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        v[i] = c0 + i < NC ? count[c0 + i] : 0;
-        a += v[i];
-    }
-    int bsum = a;
-    for (int o = 8; o > 0; o >>= 1) bsum += __shfl_xor(bsum, o, 16);
-    const bool leader = (threadIdx.x & 15) == 0;
-    int b = (leader && bsum > 0) ? 1 : 0;
-    const int nonempty = b;
-    block_scan2(a, b, s_a, s_b);
-    a += tpa;
-    b += tpb;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (c0 + i < NC) {
-            start[c0 + i] = a;
-            count[c0 + i] = 0;
-            a += v[i];
-        }
-    }
-    if (leader && c0 < NC) {
-        const long long bk = c0 >> 6;
-        slotof[bk] = nonempty ? b : -1;
-        if (nonempty) {
-            alist[b] = (int)bk;
+            for (int i = 0; i < 16; ++i) {
+                const int4 v = c4[i];
+                int4 st;
+                st.x = a; a += v.x;
+                st.y = a; a += v.y;
+                st.z = a; a += v.z;
+                st.w = a; a += v.w;
+                s4[i] = st;
+                c4[i] = make_int4(0, 0, 0, 0);
+            }
+            start[(b + 1) << 6] = a;
             const long long nbs = (long long)nb * nb * nb;
-            const long long sc = bk / nbs;
-            const int loc = (int)(bk - sc * nbs);
+            const long long sc = b / nbs;
+            const int loc = (int)(b - sc * nbs);
             const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
             for (int dx = -1; dx <= 1; ++dx)
                 for (int dy = -1; dy <= 1; ++dy)
@@ -349,11 +349,15 @@ __global__ void __launch_bounds__(WG) k_scan_apply(int* __restrict__ count, long
                     }
         }
     }
-    if (blockIdx.x == ntiles - 1 && threadIdx.x == WG - 1) {
-        start[NC] = a;
-        misc[0] = b + nonempty;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        misc[3] = misc[4];
+        misc[4] = 0;
+        misc[1] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) misc[1] = 0;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == WG - 1) {
+        misc[0] = pb + (t > 0 ? 1 : 0);
+        start[NB << 6] = pa + t;
+    }
 }
 
 // Compact the marked grid blocks into blist (any order) and clear the marks.
@@ -369,6 +373,15 @@ __global__ void __launch_bounds__(WG) k_mark_compact(int* __restrict__ mark, lon
     if (lane == __builtin_ctzll(bal)) base = atomicAdd(&misc[1], __popcll(bal));
     base = __shfl(base, __builtin_ctzll(bal));
     if (mk) blist[base + __popcll(bal & ((1ull << lane) - 1))] = (int)b;
+}
+
+// Cell starts of the empty buckets (all their cells start where the bucket
+// does) -- only for pmpm_bin's dense cell_start; the step reads the starts
+// of non-empty buckets' cells alone.
+__global__ void __launch_bounds__(WG) k_dense_starts(const int* __restrict__ bstart, const int* __restrict__ slotof,
+                                                     long long NC, int* __restrict__ start) {
+    const long long c = (long long)blockIdx.x * WG + threadIdx.x;
+    if (c < NC && slotof[c >> 6] < 0) start[c] = bstart[c >> 6];
 }
 
 __global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __restrict__ key, const int* __restrict__ slot,
@@ -1765,23 +1778,28 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     const int nb = nblk(ng);
     int* count = (int*)(ws + L.count);
     int* start = (int*)(ws + L.start);
-    int* tiles = (int*)(ws + L.tiles);
+    int* btot = (int*)(ws + L.btot);
+    int* bstart = (int*)(ws + L.bstart);
     int* alist = (int*)(ws + L.alist);
     int* slotof = (int*)(ws + L.slotof);
-    int* mark = (int*)(ws + L.mark);
+    int* blist = (int*)(ws + L.blist);
     int* key = (int*)(ws + L.key);
     int* slot = (int*)(ws + L.slot);
     int* perm = (int*)(ws + L.perm);
     int* inv = (int*)(ws + L.inv);
     int* misc = (int*)(ws + L.misc);
+    int* bflag = (int*)(ws + L.bflag);
+    int* tiles = (int*)(ws + L.tiles);
+    int* mark = (int*)(ws + L.mark);
     TimedScope ts(0, st);
     if (n > 0)
-        hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, key,
-                           slot, misc);
-    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, tiles);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, tiles, L.ntiles, misc);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)L.ntiles), dim3(WG), 0, st, count, L.NC, nb, tiles, L.ntiles, start, alist,
-                       slotof, mark, misc);
+        hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, bflag,
+                           key, slot, misc);
+    const dim3 TG((unsigned)L.ntiles);
+    hipLaunchKernelGGL(k_bucket_tiles, TG, dim3(WG), 0, st, bflag, count, L.NB, btot, tiles);
+    hipLaunchKernelGGL(k_bucket_apply, TG, dim3(WG), 0, st, bflag, count, btot, L.NB, nb, tiles, bstart, alist, slotof, mark,
+                       start, misc);
+    hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, mark, L.NB, blist, misc);
     if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm, inv);
     if (stable && n > 0)
         hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm, inv);
@@ -1884,11 +1902,6 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
                         p->n_scenes, L, ws, st);
     if (rc) return rc;
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
-    {
-        TimedScope ts(0, st);
-        hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
-                           (int*)(ws + L.blist), (int*)(ws + L.misc));
-    }
     if (p->export_active)
         hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
                            dim3(WG), 0, st, (unsigned char*)(ws + L.active), (long long)L.G);
@@ -1975,8 +1988,6 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
         // recompute the step's binning, P2G and grid pass, keeping (m, sum w)
         int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, 1, nullptr, 1, L, ws, st);
         if (rc) return rc;
-        hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, (int*)(ws + L.mark), L.NB,
-                           (int*)(ws + L.blist), (int*)(ws + L.misc));
         if (in->n > 0) launch_transfer<T>(A, p->do_damage != 0, false, st);
         hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
     }
@@ -2185,13 +2196,16 @@ int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv
     int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st)
                                : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st);
     if (rc) return rc;
-    // the grid-block marks of a bare binning call are not consumed by a grid pass
-    HIPCHK(hipMemsetAsync((char*)ws + L.mark, 0, (size_t)L.NB * 4, st));
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(perm, (char*)ws + L.perm, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
         if (key) HIPCHK(hipMemcpyAsync(key, (char*)ws + L.key, (size_t)n * 4, hipMemcpyDeviceToDevice, st));
     }
-    if (cell_start) HIPCHK(hipMemcpyAsync(cell_start, (char*)ws + L.start, (size_t)(L.NC + 1) * 4, hipMemcpyDeviceToDevice, st));
+    if (cell_start) {
+        char* w = (char*)ws;
+        hipLaunchKernelGGL(k_dense_starts, dim3(nblocks(L.NC)), dim3(WG), 0, st, (const int*)(w + L.bstart),
+                           (const int*)(w + L.slotof), L.NC, (int*)(w + L.start));
+        HIPCHK(hipMemcpyAsync(cell_start, (char*)ws + L.start, (size_t)(L.NC + 1) * 4, hipMemcpyDeviceToDevice, st));
+    }
     return PMPM_OK;
 }
 
