@@ -500,27 +500,54 @@ template <typename T> __device__ __forceinline__ void store_m3(T* __restrict__ a
         for (int j = 0; j < 3; ++j) a[9 * p + 3 * i + j] = r.m[i][j];
 }
 
+// One particle's constitutive inputs, loaded together (load_const_in) at the
+// top of a kernel so that they are all in flight at once, ahead of the
+// register-bound eigen and stress arithmetic that uses them.
+template <typename T> struct ConstIn {
+    M3<T> A0;
+    T d, mu, lam, res, sc, zeta, Dh, l0, lap, eta;
+    bool allow, iface;
+};
+template <typename T, bool DAMAGE>
+__device__ __forceinline__ void load_const_in(const StepArgs<T>& A, long long p, ConstIn<T>& q) {
+    q.d = A.d[p];
+    q.mu = A.mu[p];
+    q.lam = A.lam[p];
+    q.res = A.residual[p];
+    q.iface = A.imask[p] != 0;
+    q.Dh = A.D_hist[p];
+    if constexpr (DAMAGE) {
+        q.A0 = load_m3(A.A0, p);
+        q.sc = A.sigma_c[p];
+        q.zeta = A.zeta[p];
+        q.l0 = A.l0[p];
+        q.lap = A.lap[p];
+        q.eta = A.eta[p];
+        q.allow = A.allow[p] != 0;
+    }
+}
+
 // Stress, damage and degradation of one particle (anisompm.py:614-706).
 // Returns the degraded Kirchhoff stress; F is replaced by its clamped value
 // when f_clamp is on; d becomes the post-update damage.  DRIVE: stop after
 // the drive (phi, phi_shear and the D_hist history max are written; Dh gets
 // the new D_hist) for the implicit damage solve, which updates d itself.
 template <typename T, bool DAMAGE, bool EXACT_SIGMA, bool DRIVE = false>
-__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, M3<T>& F, T& d, T& J, bool dmg_on,
-                                              T* Dh_out = nullptr) {
+__device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p, const ConstIn<T>& q, M3<T>& F, T& d, T& J,
+                                              bool dmg_on, T* Dh_out = nullptr) {
     M3<T> Cm = mul_at(F, F);
     T lc[3];
     M3<T> V;
-    sym_eig3(Cm, lc, V);
+    sym_eig3<T, true>(Cm, lc, V);
     T S[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) S[i] = sqrt(maxp(lc[i], T(1e-12)));
     M3<T> W = mul(F, V), Wh;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        T nrm = sqrt(W.m[0][j] * W.m[0][j] + W.m[1][j] * W.m[1][j] + W.m[2][j] * W.m[2][j]) + T(1e-12);
+        const T inv = T(1) / (sqrt(W.m[0][j] * W.m[0][j] + W.m[1][j] * W.m[1][j] + W.m[2][j] * W.m[2][j]) + T(1e-12));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] / nrm;
+        for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] * inv;
     }
     M3<T> R = mul_bt(Wh, V);
     if (A.use_fclamp) {
@@ -540,7 +567,7 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         for (int i = 0; i < 3; ++i) S[i] = sc[i];
     }
     J = maxp(S[0] * S[1] * S[2], T(1e-6));
-    const T mu = A.mu[p], lam = A.lam[p];
+    const T mu = q.mu, lam = q.lam;
     M3<T> FmR;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -548,7 +575,7 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         for (int j = 0; j < 3; ++j) FmR.m[i][j] = (T(2) * mu) * (F.m[i][j] - R.m[i][j]);
     M3<T> tdev = mul_bt(FmR, F);
     const T tvol = lam * (J - T(1)) * J;
-    const bool iface = A.imask[p] != 0;
+    const bool iface = q.iface;
     T n0[3] = {T(0), T(0), T(0)};
     if (iface) {
         n0[0] = A.n_if[3 * p]; n0[1] = A.n_if[3 * p + 1]; n0[2] = A.n_if[3 * p + 2];
@@ -557,9 +584,9 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
     if (DAMAGE && !dmg_on) {
         // this scene skips damage this step: d and D_hist pass through
         A.od[p] = d;
-        A.oD[p] = A.D_hist[p];
+        A.oD[p] = q.Dh;
     } else if (DAMAGE) {
-        const T sc = A.sigma_c[p];
+        const T sc = q.sc;
         T phi, tt2 = T(0);
         if (EXACT_SIGMA) {
             // Cauchy stress, its own Jacobi, tensile part (anisompm.py:650-655)
@@ -570,14 +597,14 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
                 for (int j = 0; j < 3; ++j) sig.m[i][j] = (tdev.m[i][j] + (i == j ? tvol : T(0))) / J;
             T ev[3];
             M3<T> Q;
-            sym_eig3(sig, ev, Q);
+            sym_eig3<T, true>(sig, ev, Q);
             M3<T> QD;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) QD.m[i][j] = Q.m[i][j] * maxp(ev[j], T(0));
             M3<T> sp = mul_bt(QD, Q);
-            M3<T> A0 = load_m3(A.A0, p);
+            const M3<T>& A0 = q.A0;
             M3<T> Am = mul_bt(mul(R, A0), R);
             M3<T> M = mul(Am, sp);
             T tr = T(0);
@@ -596,10 +623,11 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
             // F = R U with U = V diag(S) V^T, so tau = (R V) diag(2mu(S-1)S + tvol) (R V)^T:
             // sigma's eigenvectors are the columns of W-hat and its eigenvalues are closed form.
             T D[3];
+            const T invJ = T(1) / J;
 #pragma unroll
-            for (int i = 0; i < 3; ++i) D[i] = maxp(((T(2) * mu) * (S[i] - T(1)) * S[i] + tvol) / J, T(0));
+            for (int i = 0; i < 3; ++i) D[i] = maxp(((T(2) * mu) * (S[i] - T(1)) * S[i] + tvol) * invJ, T(0));
             // phi = tr(A s+ A s+) = sum_ij B_ij B_ji D_i D_j with B = V^T A0 V
-            M3<T> A0 = load_m3(A.A0, p);
+            const M3<T>& A0 = q.A0;
             M3<T> B = mul_at(V, mul(A0, V));
             T tr = T(0);
 #pragma unroll
@@ -622,23 +650,23 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         A.ophi[p] = phi;
         A.ophs[p] = tt * tt;
         // history max of the drive, explicit phase-field update (anisompm.py:679-685)
-        const T Dh = maxp(A.D_hist[p], A.zeta[p] * relu(phi - T(1)));
+        const T Dh = maxp(q.Dh, q.zeta * relu(phi - T(1)));
         A.oD[p] = Dh;
         if constexpr (DRIVE) {
             *Dh_out = Dh;
             return M3<T>{};
         }
-        const T l0 = A.l0[p];
-        const T resist = d - (l0 * l0) * A.lap[p];
+        const T l0 = q.l0;
+        const T resist = d - (l0 * l0) * q.lap;
         const T ddot = relu((T(1) - d) * Dh - resist);
-        T dn = d + (A.dt_dmg / A.eta[p]) * ddot;
+        T dn = d + (A.dt_dmg / q.eta) * ddot;
         dn = minp(maxp(dn, T(0)), T(1));
-        if (A.allow[p]) d = maxp(dn, d);
+        if (q.allow) d = maxp(dn, d);
         A.od[p] = d;
     }
 
     // degradation g(d), volumetric part only in tension (anisompm.py:687-691)
-    const T res = A.residual[p];
+    const T res = q.res;
     const T om = T(1) - d;
     const T g = om * om * (T(1) - res) + res;
     const T vdeg = J >= T(1) ? g : T(1);
@@ -690,30 +718,34 @@ template <typename T, bool DAMAGE, bool EXACT_SIGMA>
 __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= A.n) return;
+    // every input of the particle is loaded here, before the arithmetic
+    M3<T> F = load_m3(A.F, p);
+    ConstIn<T> q;
+    load_const_in<T, DAMAGE>(A, p, q);
+    const M3<T> Cp = load_m3(A.C, p);
+    T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
+    const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+    const T mass = A.mass[p], vol0 = A.vol0[p];
+    const int bc = A.bc_id ? A.bc_id[p] : 0;
+    const long long i = A.inv[p];
     // a batched step gates damage per scene; the scene is in the cell key
     const bool dmg_on = !DAMAGE || A.scene_dmg == nullptr || A.scene_dmg[(A.key[p] >> 6) / A.NBs] != 0;
-    M3<T> F = load_m3(A.F, p);
-    T d = A.d[p], J;
-    M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, F, d, J, dmg_on);
+    T d = q.d, J;
+    M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, q, F, d, J, dmg_on);
     if (A.use_fclamp) store_m3(A.oF, p, F);
     store_m3(A.otau, p, tau);
     A.oJ[p] = J;
-    const T mass = A.mass[p];
-    T sk = T(-1) * A.dt * A.vol0[p];
+    T sk = T(-1) * A.dt * vol0;
     sk = sk * T(4);
     sk = sk * A.inv_dx;
     sk = sk * A.inv_dx;
-    const M3<T> Cp = load_m3(A.C, p);
     // velocity BC slot: compile-time indices only, so the by-value kernel
     // arguments are never copied to scratch for a dynamic index
-    T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
-    const int bc = A.bc_id ? A.bc_id[p] : 0;
 #pragma unroll
     for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
         if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
-    const long long i = A.inv[p];
     T* r = A.rec + (size_t)P2G_REC * i;
-    r[0] = A.x[3 * p]; r[1] = A.x[3 * p + 1]; r[2] = A.x[3 * p + 2];
+    r[0] = xp[0]; r[1] = xp[1]; r[2] = xp[2];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
@@ -1229,6 +1261,18 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const T* gvs = A.gv + 4 * scene * A.Gs;
         const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
+        const T* Fin = A.use_fclamp ? (const T*)A.oF : A.F;
+        // this thread's first particle is fetched before the node staging,
+        // so its dependent loads (perm, then x and F) overlap the staging
+        const int s0 = A.start[(long long)bk << 6], cnt = A.start[((long long)bk + 1) << 6] - s0;
+        long long p0 = 0;
+        T x0[3] = {T(0), T(0), T(0)};
+        M3<T> F0;
+        if ((int)threadIdx.x < cnt) {
+            p0 = A.perm[s0 + threadIdx.x];
+            x0[0] = A.x[3 * p0]; x0[1] = A.x[3 * p0 + 1]; x0[2] = A.x[3 * p0 + 2];
+            F0 = load_m3(Fin, p0);
+        }
         for (int l = threadIdx.x; l < AR3; l += WG) {
             const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
             const int gx = o0 + lx, gy = o1 + ly, gz = o2 + lz;
@@ -1240,10 +1284,18 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
             }
         }
         __syncthreads();
-        const int s0 = A.start[(long long)bk << 6], cnt = A.start[((long long)bk + 1) << 6] - s0;
         for (int i = threadIdx.x; i < cnt; i += WG) {
-            const long long p = A.perm[s0 + i];
-            T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+            const bool first = i == (int)threadIdx.x;
+            const long long p = first ? p0 : (long long)A.perm[s0 + i];
+            T xp[3];
+            M3<T> F;
+            if (first) {
+                xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
+                F = F0;
+            } else {
+                xp[0] = A.x[3 * p]; xp[1] = A.x[3 * p + 1]; xp[2] = A.x[3 * p + 2];
+                F = load_m3(Fin, p);
+            }
             T Xp[3] = {scaled(xp[0], A.inv_dx), scaled(xp[1], A.inv_dx), scaled(xp[2], A.inv_dx)};
             int bq[3];
             T fx[3], w[3][3];
@@ -1273,7 +1325,6 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
                 else g2p_gather_c<T, false>(s_g, bcb, org, ng, wc, w2, lap);
             }
             lap = A.lap_scale * lap;
-            const M3<T> F = load_m3(A.use_fclamp ? (const T*)A.oF : A.F, p);
             M3<T> Ip;
 #pragma unroll
             for (int a = 0; a < 3; ++a)
@@ -1364,8 +1415,10 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
             if (wv >= W || r >= cnt) continue;
             const long long p = A.perm[c0 + r];
             M3<T> F = load_m3(A.F, p);
-            T d = A.d[p], J, Dh;
-            constitutive<T, true, EXACT_SIGMA, true>(A, p, F, d, J, true, &Dh);
+            ConstIn<T> q;
+            load_const_in<T, true>(A, p, q);
+            T d = q.d, J, Dh;
+            constitutive<T, true, EXACT_SIGMA, true>(A, p, q, F, d, J, true, &Dh);
             const T c = A.dt_dmg / A.eta[p];
             const T l0 = A.l0[p];
             const T kg = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);

@@ -83,16 +83,16 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
     M3<T> Cm = mul_at(F, F);
     T lc[3];
     M3<T> V;
-    sym_eig3(Cm, lc, V);
+    sym_eig3<T, true>(Cm, lc, V);
     T S[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) S[i] = sqrt(maxp(lc[i], T(1e-12)));
     M3<T> W = mul(F, V), Wh;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const T nrm = sqrt(W.m[0][j] * W.m[0][j] + W.m[1][j] * W.m[1][j] + W.m[2][j] * W.m[2][j]) + T(1e-12);
+        const T inv = T(1) / (sqrt(W.m[0][j] * W.m[0][j] + W.m[1][j] * W.m[1][j] + W.m[2][j] * W.m[2][j]) + T(1e-12));
 #pragma unroll
-        for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] / nrm;
+        for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] * inv;
     }
     const M3<T> R = mul_bt(Wh, V);
     const T J0 = S[0] * S[1] * S[2];
@@ -128,7 +128,7 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
                 const T s = T(0.5) * (sig.m[i][j] + sig.m[j][i]);
                 sig.m[i][j] = s; sig.m[j][i] = s;
             }
-        sym_eig3(sig, ev, Q);
+        sym_eig3<T, true>(sig, ev, Q);
         M3<T> QD;
 #pragma unroll
         for (int i = 0; i < 3; ++i) fp[i] = maxp(ev[i], T(0));

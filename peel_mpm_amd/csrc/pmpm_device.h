@@ -148,19 +148,22 @@ __device__ __forceinline__ double rot_rsq(double v) { return 1.0 / sqrt(v); }
 __device__ __forceinline__ float rot_sqrt(float v) { return __builtin_amdgcn_sqrtf(v); }
 __device__ __forceinline__ double rot_sqrt(double v) { return sqrt(v); }
 
-// One Jacobi rotation zeroing a[p][q] (anisompm.py:74-91), p<q, r the third index.
-template <int p, int q, typename T> __device__ __forceinline__ void jacobi_rot(T (&a)[3][3], T (&V)[3][3]) {
+// One Jacobi rotation zeroing a[p][q] (anisompm.py:74-91), p<q, r the third
+// index.  keep: leave a and V as they are (the lane has converged).
+template <int p, int q, typename T>
+__device__ __forceinline__ void jacobi_rot(T (&a)[3][3], T (&V)[3][3], bool keep = false) {
     constexpr int r = 3 - p - q;
     const T app = a[p][p], aqq = a[q][q], apq = a[p][q], apr = a[p][r], aqr = a[q][r];
     const T zeta = (aqq - app) * rot_rcp(T(2) * apq);
     const T sgn = zeta >= T(0) ? T(1) : T(-1);
     T t = sgn * rot_rcp(fabs(zeta) + rot_sqrt(T(1) + zeta * zeta));
     t = fabs(apq) > T(1e-30) ? t : T(0);
-    const T c = rot_rsq(T(1) + t * t);
+    t = keep ? T(0) : t;
+    const T c = keep ? T(1) : rot_rsq(T(1) + t * t);
     const T s = c * t;
     a[p][p] = c * c * app - T(2) * s * c * apq + s * s * aqq;
     a[q][q] = s * s * app + T(2) * s * c * apq + c * c * aqq;
-    a[p][q] = T(0); a[q][p] = T(0);
+    a[p][q] = keep ? apq : T(0); a[q][p] = a[p][q];
     const T npr = c * apr - s * aqr, nqr = s * apr + c * aqr;
     a[p][r] = npr; a[r][p] = npr;
     a[q][r] = nqr; a[r][q] = nqr;
@@ -172,10 +175,23 @@ template <int p, int q, typename T> __device__ __forceinline__ void jacobi_rot(T
     }
 }
 
+// Off-diagonal size, relative to the diagonal, below which a Jacobi sweep
+// no longer changes a: its rotations have |t| <= ~tol / (relative gap), which
+// moves eigenvalues by ~tol^2 and eigenvectors by about one rounding unit
+// (or, for equal eigenvalues, within their common eigenspace).
+template <typename T> __device__ __forceinline__ T jacobi_tol();
+template <> __device__ __forceinline__ float jacobi_tol<float>() { return 1e-9f; }
+template <> __device__ __forceinline__ double jacobi_tol<double>() { return 1e-18; }
+
 // Symmetric 3x3 eigendecomposition by 5 cyclic Jacobi sweeps
 // (anisompm.py:48-95): symmetrise, sanitise, rotate, sort ascending; V holds
 // the eigenvectors as columns.  Ties keep their diagonal order.
-template <typename T> __device__ __forceinline__ void sym_eig3(const M3<T>& A, T ev[3], M3<T>& Vout) {
+// CONVERGED: a lane whose off-diagonal part has fallen below jacobi_tol of
+// its diagonal after a sweep takes no further rotations (a per-lane rule, so
+// the result does not depend on which particles share a wavefront), and a
+// wavefront whose lanes have all converged skips the remaining sweeps.
+template <typename T, bool CONVERGED = false>
+__device__ __forceinline__ void sym_eig3(const M3<T>& A, T ev[3], M3<T>& Vout) {
     T a[3][3], V[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -184,11 +200,20 @@ template <typename T> __device__ __forceinline__ void sym_eig3(const M3<T>& A, T
             a[i][j] = nan_to_num(T(0.5) * (A.m[i][j] + A.m[j][i]));
             V[i][j] = i == j ? T(1) : T(0);
         }
+    bool done = false;
 #pragma unroll
     for (int s = 0; s < 5; ++s) {
-        jacobi_rot<0, 1>(a, V);
-        jacobi_rot<0, 2>(a, V);
-        jacobi_rot<1, 2>(a, V);
+        if constexpr (CONVERGED) {
+            if (s > 0) {
+                const T off = fabs(a[0][1]) + fabs(a[0][2]) + fabs(a[1][2]);
+                const T dia = fabs(a[0][0]) + fabs(a[1][1]) + fabs(a[2][2]);
+                done = done || off <= jacobi_tol<T>() * dia;
+                if (__all(done)) break;
+            }
+        }
+        jacobi_rot<0, 1>(a, V, done);
+        jacobi_rot<0, 2>(a, V, done);
+        jacobi_rot<1, 2>(a, V, done);
     }
     T e[3] = {a[0][0], a[1][1], a[2][2]};
     int o[3] = {0, 1, 2};
