@@ -51,6 +51,7 @@ constexpr int BTILE = WG;              // buckets per bucket-scan tile
 constexpr int P2G_GRID = 2048;         // work-groups of the persistent P2G kernel
 constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
+constexpr int G2P_PREFETCH = 2;        // particles per G2P thread fetched before the node staging
 constexpr int P2G_REC = 16;            // values per P2G record: x(3), affine(9), mass*v(3), mass
 // PCG damage scalars (doubles in the pcgr section after the partial dots)
 enum { PCG_RZ = 0, PCG_BB, PCG_ALPHA, PCG_BETA, PCG_REL, PCG_ITER, PCG_DONE, PCG_NSCAL = 8 };
@@ -1262,16 +1263,23 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
         const T* gvs = A.gv + 4 * scene * A.Gs;
         const int o0 = BLK * bx - 1, o1 = BLK * by - 1, o2 = BLK * bz - 1;
         const T* Fin = A.use_fclamp ? (const T*)A.oF : A.F;
-        // this thread's first particle is fetched before the node staging,
-        // so its dependent loads (perm, then x and F) overlap the staging
+        // this thread's first G2P_PREFETCH particles are fetched before the
+        // node staging, so their dependent loads (perm, then x and F) overlap
+        // the staging
         const int s0 = A.start[(long long)bk << 6], cnt = A.start[((long long)bk + 1) << 6] - s0;
-        long long p0 = 0;
-        T x0[3] = {T(0), T(0), T(0)};
-        M3<T> F0;
-        if ((int)threadIdx.x < cnt) {
-            p0 = A.perm[s0 + threadIdx.x];
-            x0[0] = A.x[3 * p0]; x0[1] = A.x[3 * p0 + 1]; x0[2] = A.x[3 * p0 + 2];
-            F0 = load_m3(Fin, p0);
+        long long pf[G2P_PREFETCH];
+        T xf[G2P_PREFETCH][3];
+        M3<T> Ff[G2P_PREFETCH];
+#pragma unroll
+        for (int k = 0; k < G2P_PREFETCH; ++k) {
+            const int i = (int)threadIdx.x + k * WG;
+            pf[k] = 0;
+            xf[k][0] = xf[k][1] = xf[k][2] = T(0);
+            if (i < cnt) {
+                pf[k] = A.perm[s0 + i];
+                xf[k][0] = A.x[3 * pf[k]]; xf[k][1] = A.x[3 * pf[k] + 1]; xf[k][2] = A.x[3 * pf[k] + 2];
+                Ff[k] = load_m3(Fin, pf[k]);
+            }
         }
         for (int l = threadIdx.x; l < AR3; l += WG) {
             const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
@@ -1284,18 +1292,9 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
             }
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < cnt; i += WG) {
-            const bool first = i == (int)threadIdx.x;
-            const long long p = first ? p0 : (long long)A.perm[s0 + i];
-            T xp[3];
-            M3<T> F;
-            if (first) {
-                xp[0] = x0[0]; xp[1] = x0[1]; xp[2] = x0[2];
-                F = F0;
-            } else {
-                xp[0] = A.x[3 * p]; xp[1] = A.x[3 * p + 1]; xp[2] = A.x[3 * p + 2];
-                F = load_m3(Fin, p);
-            }
+        // one particle: velocity, affine C and the damage Laplacian from the
+        // staged nodes, then advection, the F update and the clamp
+        auto particle = [&](long long p, const T (&xp)[3], const M3<T>& F) {
             T Xp[3] = {scaled(xp[0], A.inv_dx), scaled(xp[1], A.inv_dx), scaled(xp[2], A.inv_dx)};
             int bq[3];
             T fx[3], w[3][3];
@@ -1339,6 +1338,14 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
                 A.ox[3 * p + a] = minp(maxp(xn, A.pad_lo), A.pad_hi);
             }
             A.olap[p] = lap;
+        };
+#pragma unroll
+        for (int k = 0; k < G2P_PREFETCH; ++k)
+            if ((int)threadIdx.x + k * WG < cnt) particle(pf[k], xf[k], Ff[k]);
+        for (int i = threadIdx.x + G2P_PREFETCH * WG; i < cnt; i += WG) {
+            const long long p = A.perm[s0 + i];
+            const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+            particle(p, xp, load_m3(Fin, p));
         }
         __syncthreads();
     }
