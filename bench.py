@@ -17,6 +17,10 @@ Algorithmic bytes per launch (fp32, damage on):
   G2P         152 B per particle + 16 B per active grid node
   grid update  40 B per active grid node
   binning      32 B per particle
+Whole-step rooflines (``step_roofline``, and the batched form's) use
+289 B per particle + 32 B per active quadratic node + 16 B per active cubic
+node; forward+adjoint adds 834 B per particle-step for the backward pass
+(segment recompute + adjoint kernels).
 ``cpu_baseline`` times the oracle (an op-for-op torch-CPU restatement of the
 reference step, bit-identical to it on the CPU) on this host, on a bounded
 sample of the same scene.
@@ -38,6 +42,15 @@ METRIC = "particle-steps/s at 1M fp32, batched vs plain; adjoint/forward time ra
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("binning", "stress_p2g", "grid_update", "g2p")
 PCG_GROUP = 4          # pmpm_timing group of the implicit damage solve
+
+
+STEP_B_PER_PARTICLE = 289
+ADJ_B_PER_PARTICLE = 834
+
+
+def step_roofline(step_bytes, seconds_per_step):
+    gbs = step_bytes / seconds_per_step / 1e9
+    return {"algorithmic_bytes": step_bytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
 def algorithmic_bytes(kernel, n, n_act):
@@ -119,10 +132,15 @@ def time_batched(n_scenes, steps, warmup):
     torch.cuda.synchronize()
     batched_wall = time.perf_counter() - t0
     finite = bool(torch.isfinite(b.x).all())
+    n_q, n_c = b.last_step_stats()
+    step_bytes = STEP_B_PER_PARTICLE * total + 32 * n_q + 16 * n_c
     return {"scenes": n_scenes, "particles": total, "steps": steps,
             "value": total * steps / batched_wall, "plain_scene_by_scene": total * steps / plain_wall,
             "speedup_vs_plain": plain_wall / batched_wall, "ms_per_step": batched_wall / steps * 1e3,
             "plain_ms_per_step": plain_wall / steps * 1e3, "graph": True, "finite": finite,
+            "active_nodes": {"quadratic": n_q, "cubic": n_c},
+            "step_roofline": dict(step_roofline(step_bytes, batched_wall / steps), peak=HBM_PEAK_GBS),
+            "plain_step_roofline": dict(step_roofline(step_bytes, plain_wall / steps), peak=HBM_PEAK_GBS),
             "autotune": {"config": launch, "tuning_s": round(t_tune, 2)}}
 
 
@@ -154,7 +172,15 @@ def time_adjoint(steps, every):
         t2 = time.perf_counter()
         res = {"fwd": t1 - t0, "bwd": t2 - t1}
     finite = all(bool(torch.isfinite(t.grad).all()) for t in (E, nu, a0, v0))
+    n_q, n_c = sim.last_step_stats()
+    fwd_bytes = (STEP_B_PER_PARTICLE * n + 32 * n_q + 16 * n_c) * steps
+    bwd_bytes = ADJ_B_PER_PARTICLE * n * steps
+    tot_s = res["fwd"] + res["bwd"]
     return {"workload": f"config4: {n} particles, 64^3 grid, {steps} steps, checkpoint every {every}",
+            "roofline": {"bound": "hbm", "algorithmic_bytes": fwd_bytes + bwd_bytes,
+                         "achieved_GBs": (fwd_bytes + bwd_bytes) / tot_s / 1e9, "peak": HBM_PEAK_GBS,
+                         "frac": (fwd_bytes + bwd_bytes) / tot_s / 1e9 / HBM_PEAK_GBS,
+                         "backward_frac": bwd_bytes / res["bwd"] / 1e9 / HBM_PEAK_GBS},
             "forward_ms_per_step": res["fwd"] / steps * 1e3, "backward_ms_per_step": res["bwd"] / steps * 1e3,
             "adjoint_forward_ratio": res["bwd"] / res["fwd"],
             "forward_plus_adjoint_particle_steps_per_s": n * steps / (res["fwd"] + res["bwd"]),
@@ -272,7 +298,7 @@ def main():
     n_act = max(n_q, n_c)
     alg = algorithmic_bytes(dominant, n, n_act)
     achieved = alg / (per_kernel[dominant] * 1e-3) / 1e9
-    step_bytes = 289 * n + 32 * n_q + 16 * n_c
+    step_bytes = STEP_B_PER_PARTICLE * n + 32 * n_q + 16 * n_c
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -283,8 +309,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dominant),
                      "algorithmic_bytes": alg},
         "kernel_ms": {k: round(v, 5) for k, v in per_kernel.items()},
-        "step_roofline": {"algorithmic_bytes": step_bytes, "achieved_GBs": step_bytes / (wall / args.steps) / 1e9,
-                          "frac": step_bytes / (wall / args.steps) / 1e9 / HBM_PEAK_GBS},
+        "step_roofline": step_roofline(step_bytes, wall / args.steps),
         "active_nodes": {"quadratic": n_q, "cubic": n_c},
         "autotune": tuned,
         "finite": finite,

@@ -214,3 +214,12 @@ class BatchedAnisoMPM:
     def scene(self, s: int, key: str):
         """View of attribute ``key`` restricted to scene s."""
         return getattr(self, key)[self.offsets[s]:self.offsets[s + 1]]
+
+    def last_step_stats(self):
+        """(active quadratic nodes, active cubic nodes) of the last step,
+        summed over the scenes; synchronises the stream."""
+        q, c = C.c_int64(), C.c_int64()
+        _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, self.n_scenes, self._code,
+                                                   C.byref(q), C.byref(c), _lib.stream_ptr(self.device)),
+                   "pmpm_last_step_stats")
+        return q.value, c.value
