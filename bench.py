@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
                     [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
-                    [--no-config2] [--no-tune]
+                    [--no-config2] [--no-carry] [--no-tune]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
@@ -217,6 +217,41 @@ def time_pcg(spec, steps, warmup):
             "last_residual": rel, "finite": bool(torch.isfinite(sim.d).all())}
 
 
+CARRY_B_PER_SPLAT = 180    # bind 8, x 12, F 36, d 4, off0 12, cov0 36, color 12 in; pos 12, cov 36, color 12 out
+
+
+def time_carry(spec, splats_per_particle, launches):
+    """The Gaussian carry (pmpm_gaussian_carry) after a few steps of the
+    scene: splats_per_particle splats per particle, each bound to a random
+    particle (seeded; a worst-case gather), with the crack tint on."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, scenes
+    from peel_mpm_amd.carry import carry_gaussians
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
+    for s in range(3):
+        sim.substep(s * spec["dt"])
+    g = torch.Generator(device="cuda:0").manual_seed(0)
+    G = splats_per_particle * sim.n
+    data = {"g_bind": torch.randint(0, sim.n, (G,), generator=g, device="cuda:0"),
+            "g_off0": 0.3 * sim.dx * torch.randn(G, 3, generator=g, device="cuda:0"),
+            "g_cov0": ((0.2 * sim.dx) ** 2 * torch.eye(3, device="cuda:0")).expand(G, 3, 3).contiguous(),
+            "g_color": torch.rand(G, 3, generator=g, device="cuda:0"), "grid_lim": 1.0, "scale_to_sim": 1.0,
+            "center": (0.0, 0.0, 0.0)}
+    for _ in range(2):
+        carry_gaussians(sim, data, crack_tint=0.3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        carry_gaussians(sim, data, crack_tint=0.3)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / launches
+    gbs = CARRY_B_PER_SPLAT * G / per / 1e9
+    return {"workload": f"{G} splats bound at random to the {sim.n} particles, crack tint on", "splats": G,
+            "value": G / per, "unit": "splats/s", "ms_per_launch": per * 1e3,
+            "roofline": {"bound": "hbm", "algorithmic_bytes": CARRY_B_PER_SPLAT * G, "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, three times: the
     reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the
@@ -275,6 +310,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--launch-config", default=None, metavar="JSON",
@@ -318,6 +354,8 @@ def main():
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_pcg:
         out["pcg"] = time_pcg(spec, min(args.steps, 20), 2)
+    if not args.no_carry:
+        out["gaussian_carry"] = time_carry(spec, 4, 20)
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")

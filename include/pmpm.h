@@ -253,6 +253,24 @@ const char *pmpm_version(void);
  * number of sizes written (at most n). */
 int pmpm_abi_sizes(size_t *sizes, int32_t n);
 
+/* ---- after the step: the renderer's inputs -------------------------------- */
+/* Gaussian carry, replacing the torch ops of deform_and_render
+ * (anisompm_py/src/run_orange.py:116-128): splat g rides on particle
+ * p = bind[g] (int64),
+ *   pos[g]   = (x[p] / grid_lim - 0.5) / scale_to_sim + center + F[p] off0[g]
+ *   cov[g]   = F[p] cov0[g] F[p]^T
+ *   color_out[g] = color[g] (1 - crack_tint d[p])   (color copied when crack_tint <= 0)
+ * x (N,3), F (N,3,3), d (N,) are the step's particle state; off0 (G,3),
+ * cov0 (G,3,3), color (G,3) the splats' reference offsets, covariances and
+ * colours; pos (G,3), cov (G,3,3) are written, color_out (G,3) when not NULL
+ * (d may be NULL when crack_tint <= 0).  bind must index [0, n_particles).
+ * One kernel, enqueued on stream. */
+int pmpm_gaussian_carry(int32_t dtype, int64_t n_gauss, int64_t n_particles, const int64_t *bind,
+                        const void *x, const void *F, const void *d, const void *off0,
+                        const void *cov0, const void *color, double grid_lim, double scale_to_sim,
+                        const double *center, double crack_tint, void *pos, void *cov,
+                        void *color_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

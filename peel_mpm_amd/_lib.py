@@ -23,7 +23,7 @@ EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_
            "pmpm_substep_end", "pmpm_grid_views", "pmpm_last_step_stats", "pmpm_bin", "pmpm_stencil",
            "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
            "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info",
-           "pmpm_set_launch_config", "pmpm_get_launch_config")
+           "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry")
 
 
 class Collider(C.Structure):
@@ -94,6 +94,8 @@ def lib():
     L.pmpm_abi_sizes.argtypes = [C.POINTER(sz), i32]
     L.pmpm_timing.argtypes = [i32]
     L.pmpm_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(i64), i32]
+    L.pmpm_gaussian_carry.argtypes = [i32, i64, i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, C.POINTER(dbl), dbl, vp, vp,
+                                      vp, vp]
     sizes = (sz * 4)()
     L.pmpm_abi_sizes(sizes, 4)
     mine = (C.sizeof(Collider), C.sizeof(StepParams), C.sizeof(ParticlesIn), C.sizeof(ParticlesOut))

@@ -1766,6 +1766,7 @@ __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p
 }
 
 #include "pmpm_adjoint.h"
+#include "pmpm_carry.h"
 
 // ------------------------------------------------------------------ test kernels
 template <typename T>
@@ -1978,15 +1979,16 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
         hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts, A.pscal,
                            A.pcg_tol, A.pcg_max);
-        // Iterations go out in chunks of 2, 4, 8, ... with a read of the done
+        // Iterations go out in chunks of 1, 2, 4, ... with a read of the done
         // flag between chunks, so a solve that converges in a few iterations
-        // does not pay for pcg_max_iter launches.  Under stream capture no
-        // host read is possible: all pcg_max_iter iterations are enqueued and
-        // the ones after convergence exit at their first load.
+        // does not pay for pcg_max_iter launches (nor, converging in one, for
+        // a second iteration's launches).  Under stream capture no host read
+        // is possible: all pcg_max_iter iterations are enqueued and the ones
+        // after convergence exit at their first load.
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         HIPCHK(hipStreamIsCapturing(st, &cap));
         const bool capturing = cap != hipStreamCaptureStatusNone;
-        int issued = 0, chunk = 2;
+        int issued = 0, chunk = 1;
         while (issued < p->pcg_max_iter) {
             const int k_end = capturing ? p->pcg_max_iter : std::min(p->pcg_max_iter, issued + chunk);
             for (; issued < k_end; ++issued) {
@@ -2296,6 +2298,27 @@ int pmpm_sym_eig(const void* A, int32_t dtype, int64_t n, void* evals, void* V, 
                            (float*)V);
     HIPCHK(hipGetLastError());
     return PMPM_OK;
+}
+
+int pmpm_gaussian_carry(int32_t dtype, int64_t n_gauss, int64_t n_particles, const int64_t* bind, const void* x,
+                        const void* F, const void* d, const void* off0, const void* cov0, const void* color,
+                        double grid_lim, double scale_to_sim, const double* center, double crack_tint, void* pos,
+                        void* cov, void* color_out, void* stream) {
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
+    if (n_gauss < 0 || n_particles < 0) return fail(PMPM_EARG, "%sbad size %lld", "", n_gauss < 0 ? n_gauss : n_particles);
+    if (n_gauss == 0) return PMPM_OK;
+    if (!bind || !x || !F || !off0 || !cov0 || !center || !pos || !cov || (color_out && !color) ||
+        (color_out && crack_tint > 0.0 && !d))
+        return fail(PMPM_EARG, "%snull argument", "");
+    if (n_particles == 0) return fail(PMPM_EARG, "%ssplats bound to an empty particle set", "");
+    if (!(grid_lim > 0.0) || !(scale_to_sim > 0.0))
+        return fail(PMPM_EARG, "%sgrid_lim and scale_to_sim must be positive", "");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == PMPM_F64
+               ? launch_gaussian_carry<double>(n_gauss, (const long long*)bind, x, F, d, off0, cov0, color, grid_lim,
+                                               scale_to_sim, center, crack_tint, pos, cov, color_out, st)
+               : launch_gaussian_carry<float>(n_gauss, (const long long*)bind, x, F, d, off0, cov0, color, grid_lim,
+                                              scale_to_sim, center, crack_tint, pos, cov, color_out, st);
 }
 
 }  // extern "C"
