@@ -252,6 +252,38 @@ def time_carry(spec, splats_per_particle, launches):
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
 
 
+OBS_B_PER_PARTICLE = 42     # phi, phi_shear, d 12, x 12, region + grip 2 in; phi_pp, psh_pp 16 read and written
+
+
+def time_observe(spec, frames):
+    """The peel observables (PeelObserver, one pmpm_observe_frame per frame)
+    on the scene's state after a few steps: interface = the middle fifth of
+    the slab's height, peel above it, the grip = its top tenth."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, scenes
+    from peel_mpm_amd.observe import PeelObserver
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
+    for s in range(3):
+        sim.substep(s * spec["dt"])
+    y = sim.x[:, 1]
+    t = (y - y.min()) / (y.max() - y.min())
+    interface, peel = (t > 0.4) & (t <= 0.6), t > 0.6
+    obs = PeelObserver(sim, interface, ~interface & ~peel, peel, t > 0.9, frames=frames + 2)
+    obs.observe(0)
+    obs.observe(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(frames):
+        obs.observe(2 + f)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / frames
+    gbs = OBS_B_PER_PARTICLE * sim.n / per / 1e9
+    return {"workload": f"{sim.n} particles, three regions and a grip", "ms_per_frame": per * 1e3,
+            "value": sim.n / per, "unit": "particles/s",
+            "roofline": {"bound": "hbm", "algorithmic_bytes": OBS_B_PER_PARTICLE * sim.n, "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
+
+
 def copy_calibration(mib):
     """A plain device-to-device copy of ``mib`` MiB, three times: the
     reference kernel whose FETCH_SIZE / WRITE_SIZE counters scale the
@@ -310,7 +342,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
-    ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry leg")
+    ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry and peel-observable legs")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--launch-config", default=None, metavar="JSON",
@@ -356,6 +388,7 @@ def main():
         out["pcg"] = time_pcg(spec, min(args.steps, 20), 2)
     if not args.no_carry:
         out["gaussian_carry"] = time_carry(spec, 4, 20)
+        out["peel_observables"] = time_observe(spec, 50)
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")

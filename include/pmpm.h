@@ -249,8 +249,8 @@ int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
 const char *pmpm_last_error(void);
 const char *pmpm_version(void);
 /* sizeof of pmpm_collider, pmpm_step_params, pmpm_particles_in,
- * pmpm_particles_out (in that order) for binding-layout checks; returns the
- * number of sizes written (at most n). */
+ * pmpm_particles_out, pmpm_frame_in, pmpm_frame_obs (in that order) for
+ * binding-layout checks; returns the number of sizes written (at most n). */
 int pmpm_abi_sizes(size_t *sizes, int32_t n);
 
 /* ---- after the step: the renderer's inputs -------------------------------- */
@@ -270,6 +270,44 @@ int pmpm_gaussian_carry(int32_t dtype, int64_t n_gauss, int64_t n_particles, con
                         const void *cov0, const void *color, double grid_lim, double scale_to_sim,
                         const double *center, double crack_tint, void *pos, void *cov,
                         void *color_out, void *stream);
+
+/* ---- after the step: the peel test's per-frame observables --------------- */
+/* The reductions of the reference's peel frame loop
+ * (anisompm_py/src/peel_test.py:271-310) on the device, with no host
+ * synchronisation per frame.  Per call (one frame), from the step's state:
+ *   phi_pp = max(phi_pp, phi), psh_pp = max(psh_pp, phi_shear)    (per particle)
+ *   peaks[0..2] = max(peaks[0..2], max phi over interface, flesh, peel)
+ *   peaks[3] = max(peaks[3], max phi_shear over the interface)
+ *   peaks[4] = max(peaks[4], phi_int / (max(phi_flesh, phi_peel) + 1e-9))   (routing)
+ *   onset: the first frame with mean(d[interface] > 0.5) > 0.05; on it the
+ *          snapshot buffers receive d, phi_pp, psh_pp, x, tau_last, J_last
+ *   log[10 * frame ..]: mean and max of d over the interface, the broken
+ *          fraction, max d over flesh and peel, the three region phi maxima,
+ *          mean y of the grip (cap) particles, any NaN in x (1.0 or 0.0)
+ * Maxima propagate NaN as torch's do.  phi and phi_shear may be NULL (a step
+ * without the damage update): then the running maxima and peaks stay and the
+ * log row holds NaN for the phi maxima.  Every region must hold particles,
+ * as the reference requires.  No counterpart as a function in the reference
+ * (its frame loop does these reductions with torch operations and a host
+ * read per frame). */
+typedef struct {
+    const void *phi, *phi_shear;   /* (N,) or NULL */
+    const void *d, *x;             /* (N,), (N,3) */
+    const void *tau_last, *J_last; /* (N,3,3), (N,); NULL: not snapshotted */
+    const int8_t *region;          /* (N,): 0 flesh, 1 interface, 2 peel */
+    const uint8_t *cap;            /* (N,): grip particles */
+} pmpm_frame_in;
+typedef struct {
+    void *phi_pp, *psh_pp;         /* (N,): running per-particle maxima (updated) */
+    void *d_e, *phipp_e, *pshpp_e, *x_e, *tau_e, *J_e;  /* onset snapshot (tau_e, J_e may be NULL) */
+    double *peaks;                 /* device, 5 values (start at 0) */
+    int32_t *onset;                /* device, 2 values: onset frame (start at -1), snapshot-this-frame */
+    double *log;                   /* device, 10 values per frame */
+} pmpm_frame_obs;
+/* Bytes of the per-call scratch (partial reductions) pmpm_observe_frame needs. */
+size_t pmpm_observe_workspace_bytes(void);
+int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame_in *in,
+                       const pmpm_frame_obs *obs, void *ws, size_t ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

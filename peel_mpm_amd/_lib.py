@@ -23,7 +23,8 @@ EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_
            "pmpm_substep_end", "pmpm_grid_views", "pmpm_last_step_stats", "pmpm_bin", "pmpm_stencil",
            "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
            "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info",
-           "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry")
+           "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry", "pmpm_observe_workspace_bytes",
+           "pmpm_observe_frame")
 
 
 class Collider(C.Structure):
@@ -53,6 +54,15 @@ _OUT_PTRS = ("x", "v", "F", "C", "lap", "d", "D_hist", "phi", "phi_shear", "tau_
 class LaunchConfig(C.Structure):
     _fields_ = [("bucket_wgs", C.c_int32), ("block_wgs", C.c_int32), ("p2g_pair", C.c_int32),
                 ("const_threads", C.c_int32), ("p2g_waves", C.c_int32)]
+
+
+class FrameIn(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("phi", "phi_shear", "d", "x", "tau_last", "J_last", "region", "cap")]
+
+
+class FrameObs(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("phi_pp", "psh_pp", "d_e", "phipp_e", "pshpp_e", "x_e", "tau_e", "J_e",
+                                          "peaks", "onset", "log")]
 
 
 class ParticlesIn(C.Structure):
@@ -94,11 +104,15 @@ def lib():
     L.pmpm_abi_sizes.argtypes = [C.POINTER(sz), i32]
     L.pmpm_timing.argtypes = [i32]
     L.pmpm_timing_read.argtypes = [C.POINTER(C.c_double), C.POINTER(i64), i32]
+    L.pmpm_observe_workspace_bytes.restype = sz
+    L.pmpm_observe_workspace_bytes.argtypes = []
+    L.pmpm_observe_frame.argtypes = [i32, i64, i32, C.POINTER(FrameIn), C.POINTER(FrameObs), vp, sz, vp]
     L.pmpm_gaussian_carry.argtypes = [i32, i64, i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, C.POINTER(dbl), dbl, vp, vp,
                                       vp, vp]
-    sizes = (sz * 4)()
-    L.pmpm_abi_sizes(sizes, 4)
-    mine = (C.sizeof(Collider), C.sizeof(StepParams), C.sizeof(ParticlesIn), C.sizeof(ParticlesOut))
+    sizes = (sz * 6)()
+    L.pmpm_abi_sizes(sizes, 6)
+    mine = (C.sizeof(Collider), C.sizeof(StepParams), C.sizeof(ParticlesIn), C.sizeof(ParticlesOut), C.sizeof(FrameIn),
+            C.sizeof(FrameObs))
     if tuple(sizes) != mine:
         raise ImportError(f"libpmpm ABI mismatch: library struct sizes {tuple(sizes)} != binding {mine}")
     _lib = L

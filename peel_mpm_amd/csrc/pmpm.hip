@@ -1767,6 +1767,7 @@ __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p
 
 #include "pmpm_adjoint.h"
 #include "pmpm_carry.h"
+#include "pmpm_observe.h"
 
 // ------------------------------------------------------------------ test kernels
 template <typename T>
@@ -2083,10 +2084,10 @@ const char* pmpm_last_error(void) { return g_err; }
 const char* pmpm_version(void) { return "pmpm 0.1 gfx950"; }
 
 int pmpm_abi_sizes(size_t* sizes, int32_t n) {
-    const size_t all[4] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
-                           sizeof(pmpm_particles_out)};
+    const size_t all[6] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
+                           sizeof(pmpm_particles_out), sizeof(pmpm_frame_in), sizeof(pmpm_frame_obs)};
     int k = 0;
-    for (; k < n && k < 4; ++k) sizes[k] = all[k];
+    for (; k < n && k < 6; ++k) sizes[k] = all[k];
     return k;
 }
 
@@ -2319,6 +2320,25 @@ int pmpm_gaussian_carry(int32_t dtype, int64_t n_gauss, int64_t n_particles, con
                                                scale_to_sim, center, crack_tint, pos, cov, color_out, st)
                : launch_gaussian_carry<float>(n_gauss, (const long long*)bind, x, F, d, off0, cov0, color, grid_lim,
                                               scale_to_sim, center, crack_tint, pos, cov, color_out, st);
+}
+
+size_t pmpm_observe_workspace_bytes(void) { return (size_t)OBS_WG_COUNT * OBS_NSLOT * sizeof(double); }
+
+int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame_in* in, const pmpm_frame_obs* obs, void* ws,
+                       size_t ws_bytes, void* stream) {
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
+    if (!in || !obs || !ws) return fail(PMPM_EARG, "%snull argument", "");
+    if (n <= 0) return fail(PMPM_EARG, "%sno particles: %lld", "", n);
+    if (frame < 0) return fail(PMPM_EARG, "%snegative frame %lld", "", frame);
+    if (!in->d || !in->x || !in->region || !in->cap || !obs->peaks || !obs->onset || !obs->log || !obs->d_e ||
+        !obs->x_e || !obs->phipp_e || !obs->pshpp_e || !obs->phi_pp || !obs->psh_pp)
+        return fail(PMPM_EARG, "%snull argument", "");
+    if ((obs->tau_e && !in->tau_last) || (obs->J_e && !in->J_last)) return fail(PMPM_EARG, "%snull argument", "");
+    if (ws_bytes < pmpm_observe_workspace_bytes())
+        return fail(PMPM_ESIZE, "%sobserve workspace too small, need %lld bytes", "", (long long)pmpm_observe_workspace_bytes());
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == PMPM_F64 ? launch_observe<double>(n, frame, in, obs, (double*)ws, st)
+                             : launch_observe<float>(n, frame, in, obs, (double*)ws, st);
 }
 
 }  // extern "C"
