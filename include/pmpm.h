@@ -182,7 +182,9 @@ typedef struct {
     int32_t const_threads;  /* threads per work-group of the per-particle constitutive
                                kernel: 64, 128 or 256; default 256 */
     int32_t p2g_waves;      /* wavefronts (each with its own LDS arena) per bucket in P2G:
-                               2 or 4 (fp32; fp64 always uses 2); default 2 */
+                               1, 2 or 4 (4: fp32 only, fp64 takes 2); default 2 */
+    int32_t g2p_threads;    /* threads per work-group (one bucket) of G2P: 128, 256 or 512;
+                               default 256 */
 } pmpm_launch_config;
 /* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
 int pmpm_set_launch_config(const pmpm_launch_config *cfg);

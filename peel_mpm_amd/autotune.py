@@ -14,7 +14,8 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   (2: smaller work-groups, more of them per CU; 4: more particles of a
   bucket in flight);
 * ``const_threads``: work-group size of the per-particle constitutive
-  kernel.
+  kernel;
+* ``g2p_threads``: work-group size of G2P (one bucket per work-group).
 
 The bucket itself (4x4x4 cells, one cell per lane of a 64-wide wavefront)
 is the unit of the data layout, not a launch parameter.
@@ -38,12 +39,14 @@ from . import _lib
 
 SPACE = {
     "bucket_wgs": (2048, 4096, 8192),
-    "p2g_waves": (2, 4),
+    "p2g_waves": (1, 2, 4),
+    "g2p_threads": (128, 256, 512),
     "p2g_pair": (1, 2),
     "const_threads": (64, 128, 256),
     "block_wgs": (512, 1024, 2048),
 }
-DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2}
+DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2,
+           "g2p_threads": 256}
 _CACHE: dict = {}
 
 

@@ -110,7 +110,7 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
-pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2};
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256};
 
 // Workspace layout (byte offsets).  The step keeps count, bflag and mark at
 // zero between steps: whoever reads them clears them.  count/start are per
@@ -1251,8 +1251,8 @@ __device__ __forceinline__ void g2p_gather_c(const T (*s_g)[AR3], const int bc[3
 }
 
 // G2P + cubic Laplacian + advection/F update/clamp per bucket (anisompm.py:748-803).
-template <typename T>
-__global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
+template <typename T, int NT>
+__global__ void __launch_bounds__(NT) k_g2p(StepArgs<T> A) {
     __shared__ T s_g[4][AR3];
     const int nact = A.misc[0];
     const int nb = A.nb, ng = A.ng;
@@ -1272,7 +1272,7 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
         M3<T> Ff[G2P_PREFETCH];
 #pragma unroll
         for (int k = 0; k < G2P_PREFETCH; ++k) {
-            const int i = (int)threadIdx.x + k * WG;
+            const int i = (int)threadIdx.x + k * NT;
             pf[k] = 0;
             xf[k][0] = xf[k][1] = xf[k][2] = T(0);
             if (i < cnt) {
@@ -1281,7 +1281,7 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
                 Ff[k] = load_m3(Fin, pf[k]);
             }
         }
-        for (int l = threadIdx.x; l < AR3; l += WG) {
+        for (int l = threadIdx.x; l < AR3; l += NT) {
             const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
             const int gx = o0 + lx, gy = o1 + ly, gz = o2 + lz;
             if (gx < 0 || gy < 0 || gz < 0 || gx >= ng || gy >= ng || gz >= ng) {
@@ -1341,8 +1341,8 @@ __global__ void __launch_bounds__(WG) k_g2p(StepArgs<T> A) {
         };
 #pragma unroll
         for (int k = 0; k < G2P_PREFETCH; ++k)
-            if ((int)threadIdx.x + k * WG < cnt) particle(pf[k], xf[k], Ff[k]);
-        for (int i = threadIdx.x + G2P_PREFETCH * WG; i < cnt; i += WG) {
+            if ((int)threadIdx.x + k * NT < cnt) particle(pf[k], xf[k], Ff[k]);
+        for (int i = threadIdx.x + G2P_PREFETCH * NT; i < cnt; i += NT) {
             const long long p = A.perm[s0 + i];
             const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
             particle(p, xp, load_m3(Fin, p));
@@ -1828,6 +1828,11 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
             return;
         }
     }
+    if (g_launch.p2g_waves == 1) {
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1, 1>), G, dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g<T, 2, 1>), G, dim3(64), 0, st, A);
+        return;
+    }
     {
         if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1, 2>), G, dim3(128), 0, st, A);
         else hipLaunchKernelGGL((k_p2g<T, 2, 2>), G, dim3(128), 0, st, A);
@@ -2035,7 +2040,10 @@ int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
     if (in->n > 0) {
         TimedScope ts(3, st);
-        hipLaunchKernelGGL(k_g2p<T>, dim3(g_launch.bucket_wgs), dim3(WG), 0, st, A);
+        const dim3 G(g_launch.bucket_wgs);
+        if (g_launch.g2p_threads == 128) hipLaunchKernelGGL((k_g2p<T, 128>), G, dim3(128), 0, st, A);
+        else if (g_launch.g2p_threads == 512) hipLaunchKernelGGL((k_g2p<T, 512>), G, dim3(512), 0, st, A);
+        else hipLaunchKernelGGL((k_g2p<T, 256>), G, dim3(256), 0, st, A);
     }
     HIPCHK(hipGetLastError());
     return PMPM_OK;
@@ -2130,13 +2138,14 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
 }
 
 int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
-    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2};
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256};
     if (cfg) {
         if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
         if (cfg->block_wgs) c.block_wgs = cfg->block_wgs;
         if (cfg->p2g_pair) c.p2g_pair = cfg->p2g_pair;
         if (cfg->const_threads) c.const_threads = cfg->const_threads;
         if (cfg->p2g_waves) c.p2g_waves = cfg->p2g_waves;
+        if (cfg->g2p_threads) c.g2p_threads = cfg->g2p_threads;
     }
     if (c.bucket_wgs < 8 || c.bucket_wgs > 16384 || c.bucket_wgs % 8 != 0)
         return fail(PMPM_EARG, "%sbucket_wgs must be a multiple of 8 in [8, 16384]: %lld", "", c.bucket_wgs);
@@ -2145,7 +2154,10 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
     if (c.p2g_pair != 1 && c.p2g_pair != 2) return fail(PMPM_EARG, "%sp2g_pair must be 1 or 2: %lld", "", c.p2g_pair);
     if (c.const_threads != 64 && c.const_threads != 128 && c.const_threads != 256)
         return fail(PMPM_EARG, "%sconst_threads must be 64, 128 or 256: %lld", "", c.const_threads);
-    if (c.p2g_waves != 2 && c.p2g_waves != 4) return fail(PMPM_EARG, "%sp2g_waves must be 2 or 4: %lld", "", c.p2g_waves);
+    if (c.p2g_waves != 1 && c.p2g_waves != 2 && c.p2g_waves != 4)
+        return fail(PMPM_EARG, "%sp2g_waves must be 1, 2 or 4: %lld", "", c.p2g_waves);
+    if (c.g2p_threads != 128 && c.g2p_threads != 256 && c.g2p_threads != 512)
+        return fail(PMPM_EARG, "%sg2p_threads must be 128, 256 or 512: %lld", "", c.g2p_threads);
     g_launch = c;
     return PMPM_OK;
 }

@@ -30,7 +30,8 @@ def run(cfg, steps=3, **kw):
 def test_work_group_counts_leave_the_step_bit_identical():
     ref = run(None)
     for cfg in ({"bucket_wgs": 1024, "block_wgs": 512}, {"bucket_wgs": 4096, "const_threads": 64},
-                {"block_wgs": 1, "bucket_wgs": 8, "const_threads": 128}):
+                {"block_wgs": 1, "bucket_wgs": 8, "const_threads": 128}, {"g2p_threads": 128},
+                {"g2p_threads": 512, "bucket_wgs": 8}):
         # (the number of P2G waves changes how a bucket's particles are split
         # among per-wave arenas, hence the summation order: compared below)
         sim = run(cfg)
@@ -40,7 +41,8 @@ def test_work_group_counts_leave_the_step_bit_identical():
 
 def test_single_particle_p2g_matches_paired_to_rounding():
     a = run({"p2g_pair": 2})
-    for cfg in ({"p2g_pair": 1}, {"p2g_waves": 4}, {"p2g_waves": 4, "p2g_pair": 1}):
+    for cfg in ({"p2g_pair": 1}, {"p2g_waves": 4}, {"p2g_waves": 4, "p2g_pair": 1}, {"p2g_waves": 1},
+                {"p2g_waves": 1, "p2g_pair": 1}):
         b = run(cfg)
         for k in ("x", "v", "F", "C"):
             assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, (cfg, k)
