@@ -1397,16 +1397,218 @@ template <int NCH> __device__ __forceinline__ void store_partials(double v[NCH],
     }
 }
 
-// Drive + assembly per bucket: the constitutive update up to the D_hist
-// history max (phi, phi_shear, D_hist written), then the cubic scatter of
-// (N mcoef, N bcoef, kappa |grad N|^2, N) into the bucket's arena slot, lane
-// per cell as in P2G.
+// Drive per particle, in particle order (the particle arrays are read
+// coalesced): the constitutive update up to the D_hist history max (phi,
+// phi_shear, D_hist written), then the particle's solve record at its sorted
+// position -- x, kappa = c l0^2 inv_dx^2, the mass coefficient
+// 1 + c (1 + D_hist) and the right-hand side d + c D_hist (c = dt_dmg / eta)
+// -- and, for the interpolation after the solve, the old d and the allow
+// flag, in the P2G record buffer, which the step fills only after the solve.
+constexpr int PCG_REC = 8;              // values per solve record: x(3), kappa, mcoef, bcoef, d, allow
 template <typename T, bool EXACT_SIGMA>
+__global__ void __launch_bounds__(WG) k_pcg_const(StepArgs<T> A) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= A.n) return;
+    M3<T> F = load_m3(A.F, p);
+    ConstIn<T> q;
+    load_const_in<T, true>(A, p, q);
+    const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+    const T l0 = A.l0[p];
+    const long long i = A.inv[p];
+    T d = q.d, J, Dh;
+    constitutive<T, true, EXACT_SIGMA, true>(A, p, q, F, d, J, true, &Dh);
+    const T c = A.dt_dmg / q.eta;
+    T* r = A.rec + (size_t)PCG_REC * i;
+    r[0] = xp[0]; r[1] = xp[1]; r[2] = xp[2];
+    r[3] = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);
+    r[4] = T(1) + c * (T(1) + Dh);
+    r[5] = d + c * Dh;
+    r[6] = q.d;
+    r[7] = q.allow ? T(1) : T(0);
+}
+
+// One particle's solve record unpacked for the cubic stencil: weights and
+// slopes per axis, kappa, the two coefficients and the frame's base node.
+template <typename T> struct PcgPart {
+    T wc[4][3], ds[4][3], kg, mco, bco;
+    int bcb[3];
+};
+
+template <typename T, bool COEF>
+__device__ __forceinline__ void pcg_load(const StepArgs<T>& A, long long i, int ng, PcgPart<T>& q) {
+    const T* rc = (const T*)__builtin_assume_aligned(A.rec + (size_t)PCG_REC * i, 4 * sizeof(T));
+    const T Xp[3] = {scaled(rc[0], A.inv_dx), scaled(rc[1], A.inv_dx), scaled(rc[2], A.inv_dx)};
+    q.kg = rc[3];
+    if (COEF) { q.mco = rc[4]; q.bco = rc[5]; }
+    T w2[4][3];
+    cubic_weights(Xp, ng, q.bcb, q.wc, w2);
+    cubic_slopes(Xp, q.ds);
+}
+
+// A second particle that does not share the first one's frame contributes
+// nothing to a pair's pass (it is scattered alone afterwards).
+template <typename T> __device__ __forceinline__ void pcg_mute(PcgPart<T>& q) {
+    q.kg = T(0); q.mco = T(0); q.bco = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { q.wc[k][a] = T(0); q.ds[k][a] = T(0); }
+}
+
+// Scatter NP particles that share one frame into one wavefront's arena of
+// (N mcoef, N bcoef, kappa |grad N|^2, N) per node: the NP contributions
+// are summed in registers, then one read-modify-write per node, in program
+// order, software-pipelined as the P2G scatter (the next node's entry is
+// read right after this node's write).
+template <typename T, int NP>
+__device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], const int lp[3][4], const PcgPart<T> (&q)[NP]) {
+    T c0, c1, c2, c3;
+    {
+        const int l = (lp[0][0] * AR + lp[1][0]) * AR + lp[2][0];
+        c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            T wxy[NP], gxy[NP], gyx[NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                wxy[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
+                gxy[j] = q[j].ds[ox][0] * q[j].wc[oy][1];
+                gyx[j] = q[j].wc[ox][0] * q[j].ds[oy][1];
+            }
+            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz) {
+                T a0, a1, a2, a3;
+#pragma unroll
+                for (int j = 0; j < NP; ++j) {
+                    const T nw = wxy[j] * q[j].wc[oz][2];
+                    const T gx = gxy[j] * q[j].wc[oz][2];
+                    const T gy = gyx[j] * q[j].wc[oz][2];
+                    const T gz = wxy[j] * q[j].ds[oz][2];
+                    const T t0 = nw * q[j].mco, t1 = nw * q[j].bco, t2 = q[j].kg * (gx * gx + gy * gy + gz * gz);
+                    if (j == 0) { a0 = t0; a1 = t1; a2 = t2; a3 = nw; }
+                    else { a0 += t0; a1 += t1; a2 += t2; a3 += nw; }
+                }
+                const int l = lxy + lp[2][oz];
+                c0 += a0; c1 += a1; c2 += a2; c3 += a3;
+                sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3;
+                __asm__ __volatile__("" ::: "memory");
+                const int n = (ox * 4 + oy) * 4 + oz + 1;
+                if (n < 64) {
+                    const int ln = (lp[0][n >> 4] * AR + lp[1][(n >> 2) & 3]) * AR + lp[2][n & 3];
+                    c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+}
+
+// K p for NP particles that share one frame: gather g_j = kappa_j sum_i
+// grad N_ij p_i from the staged search direction (one LDS read per node for
+// the NP particles), then scatter sum_j grad N_ij . g_j into one wavefront's
+// arena, software-pipelined as above.  Both halves are separable: per
+// (ox, oy) the gather keeps sum_z N_z p and sum_z N'_z p, and the scatter
+// forms N_z (N'_x N_y g_x + N_x N'_y g_y) + N'_z (N_x N_y g_z).
+template <typename T, int NP>
+__device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, const int lp[3][4], const PcgPart<T> (&q)[NP]) {
+    T g[NP][3];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) g[j][0] = g[j][1] = g[j][2] = T(0);
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+            T pz[4];
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz) pz[oz] = sp[lxy + lp[2][oz]];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                T s = q[j].wc[0][2] * pz[0], t = q[j].ds[0][2] * pz[0];
+#pragma unroll
+                for (int oz = 1; oz < 4; ++oz) { s += q[j].wc[oz][2] * pz[oz]; t += q[j].ds[oz][2] * pz[oz]; }
+                g[j][0] += (q[j].ds[ox][0] * q[j].wc[oy][1]) * s;
+                g[j][1] += (q[j].wc[ox][0] * q[j].ds[oy][1]) * s;
+                g[j][2] += (q[j].wc[ox][0] * q[j].wc[oy][1]) * t;
+            }
+            // at most one row of reads in flight (bounds the live registers)
+            if (oy == 3) __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) { g[j][0] *= q[j].kg; g[j][1] *= q[j].kg; g[j][2] *= q[j].kg; }
+    T c = sa[(lp[0][0] * AR + lp[1][0]) * AR + lp[2][0]];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+        for (int oy = 0; oy < 4; ++oy) {
+            T u[NP], v[NP];
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                // associated unlike the gather's products, which therefore
+                // need not stay live across the two halves
+                u[j] = q[j].ds[ox][0] * (q[j].wc[oy][1] * g[j][0]) + q[j].wc[ox][0] * (q[j].ds[oy][1] * g[j][1]);
+                v[j] = q[j].wc[ox][0] * (q[j].wc[oy][1] * g[j][2]);
+            }
+            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+#pragma unroll
+            for (int oz = 0; oz < 4; ++oz) {
+                T a = q[0].wc[oz][2] * u[0] + q[0].ds[oz][2] * v[0];
+#pragma unroll
+                for (int j = 1; j < NP; ++j) a += q[j].wc[oz][2] * u[j] + q[j].ds[oz][2] * v[j];
+                c += a;
+                sa[lxy + lp[2][oz]] = c;
+                __asm__ __volatile__("" ::: "memory");
+                const int n = (ox * 4 + oy) * 4 + oz + 1;
+                if (n < 64) c = sa[(lp[0][n >> 4] * AR + lp[1][(n >> 2) & 3]) * AR + lp[2][n & 3]];
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+}
+
+// One wavefront's pass over a cell's particles r, r + 1 (pairs that share a
+// frame; a second particle beyond the grid, whose clamped frame differs, is
+// scattered alone).
+template <typename T, bool APPLY>
+__device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, bool two, const int org[3], const T* sp, T* arena) {
+    PcgPart<T> q[2];
+    pcg_load<T, !APPLY>(A, i0, A.ng, q[0]);
+    pcg_load<T, !APPLY>(A, i0 + (two ? 1 : 0), A.ng, q[1]);
+    int lp[3][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+    const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
+    if (!same) pcg_mute(q[1]);
+    if (APPLY) pcg_apply_scatter<T, 2>(sp, arena, lp, q);
+    else pcg_drive_scatter<T, 2>((T(*)[4])arena, lp, q);
+    if (two && !same) {
+        PcgPart<T> q1[1];
+        pcg_load<T, !APPLY>(A, i0 + 1, A.ng, q1[0]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+        if (APPLY) pcg_apply_scatter<T, 1>(sp, arena, lp, q1);
+        else pcg_drive_scatter<T, 1>((T(*)[4])arena, lp, q1);
+    }
+}
+
+// Assembly per bucket from the solve records: the cubic scatter of
+// (N mcoef, N bcoef, kappa |grad N|^2, N) into the bucket's arena slot, lane
+// per cell as in P2G, two particles of a cell per pass (wavefront w:
+// particles 2w, 2w + 1, then 2(w + W), ...).
+template <typename T>
 __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
     constexpr int W = sizeof(T) == 4 ? 4 : 2;
     __shared__ __align__(16) T s_q[W][AR3][4];
     const int nact = A.misc[0];
-    const int nb = A.nb, ng = A.ng;
+    const int nb = A.nb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
@@ -1418,50 +1620,9 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
-        for (int r = wv; r < rounds; r += W) {
+        for (int r = 2 * wv; r < rounds; r += 2 * W) {
             if (wv >= W || r >= cnt) continue;
-            const long long p = A.perm[c0 + r];
-            M3<T> F = load_m3(A.F, p);
-            ConstIn<T> q;
-            load_const_in<T, true>(A, p, q);
-            T d = q.d, J, Dh;
-            constitutive<T, true, EXACT_SIGMA, true>(A, p, q, F, d, J, true, &Dh);
-            const T c = A.dt_dmg / A.eta[p];
-            const T l0 = A.l0[p];
-            const T kg = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);
-            const T mco = T(1) + c * (T(1) + Dh), bco = d + c * Dh;
-            const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx),
-                             scaled(A.x[3 * p + 2], A.inv_dx)};
-            int bcb[3];
-            T wc[4][3], w2[4][3], ds[4][3];
-            cubic_weights(Xp, ng, bcb, wc, w2);
-            cubic_slopes(Xp, ds);
-            int lp[3][4];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
-#pragma unroll
-            for (int ox = 0; ox < 4; ++ox)
-#pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
-#pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const T nw = wc[ox][0] * wc[oy][1] * wc[oz][2];
-                        const T gx = ds[ox][0] * wc[oy][1] * wc[oz][2];
-                        const T gy = wc[ox][0] * ds[oy][1] * wc[oz][2];
-                        const T gz = wc[ox][0] * wc[oy][1] * ds[oz][2];
-                        T* q = s_q[wv][lxy + lp[2][oz]];
-                        T q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-                        q0 += nw * mco;
-                        q1 += nw * bco;
-                        q2 += kg * (gx * gx + gy * gy + gz * gz);
-                        q3 += nw;
-                        q[0] = q0; q[1] = q1; q[2] = q2; q[3] = q3;
-                        __asm__ __volatile__("" ::: "memory");
-                    }
-                }
+            pcg_pair<T, false>(A, c0 + r, r + 1 < cnt, org, nullptr, &s_q[wv][0][0]);
         }
         __syncthreads();
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
@@ -1478,8 +1639,8 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
 
 // K p through the particles, per bucket: stage the bucket's 343 nodes of the
 // search direction (p_new = r / diag + beta p, or p itself when DIRECT) in
-// LDS, then per particle  g = sum_i grad N_i p_i  and scatter
-// kappa grad N_i . g  into the arena slot (channel 0).
+// LDS, then per pair of a cell's particles g = kappa sum_i grad N_i p_i and
+// the scatter of grad N_i . g into the arena slot (channel 0).
 template <typename T, bool DIRECT>
 __global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
     if (A.pscal[PCG_DONE] != 0.0) return;
@@ -1508,53 +1669,9 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
-        for (int r = wv; r < rounds; r += W) {
+        for (int r = 2 * wv; r < rounds; r += 2 * W) {
             if (wv >= W || r >= cnt) continue;
-            const long long p = A.perm[c0 + r];
-            const T c = A.dt_dmg / A.eta[p];
-            const T l0 = A.l0[p];
-            const T kg = (c * (l0 * l0)) * (A.inv_dx * A.inv_dx);
-            const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx),
-                             scaled(A.x[3 * p + 2], A.inv_dx)};
-            int bcb[3];
-            T wc[4][3], w2[4][3], ds[4][3];
-            cubic_weights(Xp, ng, bcb, wc, w2);
-            cubic_slopes(Xp, ds);
-            int lp[3][4];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
-            // gather the gradient of p (grid units)
-            T g0 = T(0), g1 = T(0), g2 = T(0);
-#pragma unroll
-            for (int ox = 0; ox < 4; ++ox)
-#pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
-#pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const T pv = s_p[lxy + lp[2][oz]];
-                        g0 += (ds[ox][0] * wc[oy][1] * wc[oz][2]) * pv;
-                        g1 += (wc[ox][0] * ds[oy][1] * wc[oz][2]) * pv;
-                        g2 += (wc[ox][0] * wc[oy][1] * ds[oz][2]) * pv;
-                    }
-                }
-            g0 *= kg; g1 *= kg; g2 *= kg;
-#pragma unroll
-            for (int ox = 0; ox < 4; ++ox)
-#pragma unroll
-                for (int oy = 0; oy < 4; ++oy) {
-                    const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
-#pragma unroll
-                    for (int oz = 0; oz < 4; ++oz) {
-                        const T kv = (ds[ox][0] * wc[oy][1] * wc[oz][2]) * g0 + (wc[ox][0] * ds[oy][1] * wc[oz][2]) * g1 +
-                                     (wc[ox][0] * wc[oy][1] * ds[oz][2]) * g2;
-                        T* q = &s_a[wv][lxy + lp[2][oz]];
-                        *q = *q + kv;
-                        __asm__ __volatile__("" ::: "memory");
-                    }
-                }
+            pcg_pair<T, true>(A, c0 + r, r + 1 < cnt, org, s_p, s_a[wv]);
         }
         __syncthreads();
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
@@ -1730,29 +1847,55 @@ __global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ pa
 
 // New particle damage: the interpolant of the nodal solution, clamped to
 // [0, 1] and made irreversible where allowed, as the explicit update's value
-// is (anisompm.py:684-685).  Particles in sorted order, so neighbours share
-// nodes in the caches.
+// is (anisompm.py:684-685).  Per bucket: the bucket's 343 nodes of the
+// solution staged in LDS (positions beyond the grid read the edge node, the
+// reference's clamped node index), then lane per cell over the solve
+// records, which also carry the old d and the allow flag.
 template <typename T>
-__global__ void __launch_bounds__(WG) k_pcg_interp(StepArgs<T> A) {
-    const long long i = (long long)blockIdx.x * WG + threadIdx.x;
-    if (i >= A.n) return;
-    const long long p = A.perm[i];
-    const int ng = A.ng;
-    const T Xp[3] = {scaled(A.x[3 * p], A.inv_dx), scaled(A.x[3 * p + 1], A.inv_dx), scaled(A.x[3 * p + 2], A.inv_dx)};
-    int bcb[3];
-    T wc[4][3], w2[4][3];
-    cubic_weights(Xp, ng, bcb, wc, w2);
-    T di = T(0);
+__global__ void __launch_bounds__(P2G_WG) k_pcg_interp(StepArgs<T> A) {
+    constexpr int W = P2G_WG / 64;
+    __shared__ T s_x[AR3];
+    const int nact = A.misc[0];
+    const int nb = A.nb, ng = A.ng;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
+        for (int l = threadIdx.x; l < AR3; l += P2G_WG) {
+            const int lx = l / (AR * AR), ly = (l / AR) % AR, lz = l % AR;
+            s_x[l] = A.gv[4 * node_lin(org[0] + lx, org[1] + ly, org[2] + lz, ng)];
+        }
+        __syncthreads();
+        const long long cell = ((long long)bk << 6) + lane;
+        const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
+        for (int r = wv; r < cnt; r += W) {
+            const T* rc = (const T*)__builtin_assume_aligned(A.rec + (size_t)PCG_REC * (c0 + r), 4 * sizeof(T));
+            const T Xp[3] = {scaled(rc[0], A.inv_dx), scaled(rc[1], A.inv_dx), scaled(rc[2], A.inv_dx)};
+            const T dold = rc[6];
+            const bool allow = rc[7] != T(0);
+            const long long p = A.perm[c0 + r];
+            int bcb[3];
+            T wc[4][3], w2[4][3];
+            cubic_weights(Xp, ng, bcb, wc, w2);
+            int lp[3][4];
 #pragma unroll
-    for (int ox = 0; ox < 4; ++ox)
+            for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int oy = 0; oy < 4; ++oy)
+                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, AR - 1);
+            T di = T(0);
 #pragma unroll
-            for (int oz = 0; oz < 4; ++oz)
-                di += (wc[ox][0] * wc[oy][1] * wc[oz][2]) * A.gv[4 * node_lin(bcb[0] + ox, bcb[1] + oy, bcb[2] + oz, ng)];
-    const T dold = A.d[p];
-    const T dn = minp(maxp(di, T(0)), T(1));
-    A.od[p] = A.allow[p] ? maxp(dn, dold) : dold;
+            for (int ox = 0; ox < 4; ++ox)
+#pragma unroll
+                for (int oy = 0; oy < 4; ++oy)
+#pragma unroll
+                    for (int oz = 0; oz < 4; ++oz)
+                        di += (wc[ox][0] * wc[oy][1] * wc[oz][2]) * s_x[(lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz]];
+            const T dn = minp(maxp(di, T(0)), T(1));
+            A.od[p] = allow ? maxp(dn, dold) : dold;
+        }
+        __syncthreads();
+    }
 }
 
 // Zero n bytes at a 16-byte aligned p (a fill kernel rather than
@@ -1977,8 +2120,11 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         TimedScope ts(4, st);
         const dim3 PG(g_launch.bucket_wgs), PB(P2G_WG), GG(g_launch.block_wgs), GB(WG);
         const int nparts = g_launch.block_wgs;
-        if (exact_sigma) hipLaunchKernelGGL((k_pcg_drive<T, true>), PG, PB, 0, st, A);
-        else hipLaunchKernelGGL((k_pcg_drive<T, false>), PG, PB, 0, st, A);
+        const int ct = g_launch.const_threads;
+        const dim3 CG((unsigned)((A.n + ct - 1) / ct)), CB(ct);
+        if (exact_sigma) hipLaunchKernelGGL((k_pcg_const<T, true>), CG, CB, 0, st, A);
+        else hipLaunchKernelGGL((k_pcg_const<T, false>), CG, CB, 0, st, A);
+        hipLaunchKernelGGL(k_pcg_drive<T>, PG, PB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_ASSEMBLE>), GG, GB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY_DIRECT>), GG, GB, 0, st, A);
@@ -2013,7 +2159,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             if (flag != 0.0) break;
             chunk *= 2;
         }
-        hipLaunchKernelGGL(k_pcg_interp<T>, dim3(nblocks(in->n)), GB, 0, st, A);
+        hipLaunchKernelGGL(k_pcg_interp<T>, PG, PB, 0, st, A);
     }
     if (in->n > 0) {
         TimedScope ts(1, st);
