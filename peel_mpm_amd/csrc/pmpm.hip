@@ -9,8 +9,9 @@
 //                 buckets: bucket and cell offsets, ordered list of
 //                 non-empty buckets, marks on the grid blocks their
 //                 stencils reach
-//   k_mark_compact list of marked grid blocks (clears the marks)
-//   k_bin_scatter perm[start[key]+slot] = particle and its inverse inv
+//   k_bin_scatter perm[start[key]+slot] = particle and its inverse inv; its
+//                 first work-groups list the marked grid blocks (clearing
+//                 the marks)
 //                 (+ k_bin_stabilize: ascending particle ids per cell)
 //   k_const       per particle, in particle order: stress/damage/degradation
 //                 in registers, then the particle's P2G record (x, affine
@@ -361,10 +362,11 @@ __global__ void __launch_bounds__(WG) k_bucket_apply(int* __restrict__ bflag, in
     }
 }
 
-// Compact the marked grid blocks into blist (any order) and clear the marks.
-__global__ void __launch_bounds__(WG) k_mark_compact(int* __restrict__ mark, long long NB, int* __restrict__ blist,
-                                                     int* __restrict__ misc) {
-    const long long b = (long long)blockIdx.x * WG + threadIdx.x;
+// Compact the marked grid blocks into blist (any order) and clear the marks
+// (work-group wg of the launch).
+__device__ __forceinline__ void mark_compact(long long wg, int* __restrict__ mark, long long NB, int* __restrict__ blist,
+                                             int* __restrict__ misc) {
+    const long long b = wg * WG + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool mk = b < NB && mark[b] != 0;
     if (mk) mark[b] = 0;
@@ -385,10 +387,18 @@ __global__ void __launch_bounds__(WG) k_dense_starts(const int* __restrict__ bst
     if (c < NC && slotof[c >> 6] < 0) start[c] = bstart[c >> 6];
 }
 
+// One launch for two independent passes: the first nmark work-groups compact
+// the marked grid blocks, the others place each particle at start[key] +
+// slot (perm and its inverse inv).
 __global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __restrict__ key, const int* __restrict__ slot,
                                                     const int* __restrict__ start, int* __restrict__ perm,
-                                                    int* __restrict__ inv) {
-    long long p = (long long)blockIdx.x * WG + threadIdx.x;
+                                                    int* __restrict__ inv, long long nmark, int* __restrict__ mark,
+                                                    long long NB, int* __restrict__ blist, int* __restrict__ misc) {
+    if ((long long)blockIdx.x < nmark) {
+        mark_compact(blockIdx.x, mark, NB, blist, misc);
+        return;
+    }
+    const long long p = ((long long)blockIdx.x - nmark) * WG + threadIdx.x;
     if (p < n) {
         const int pos = start[key[p]] + slot[p];
         perm[pos] = (int)p;
@@ -2009,8 +2019,9 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     hipLaunchKernelGGL(k_bucket_tiles, TG, dim3(WG), 0, st, bflag, count, L.NB, btot, tiles);
     hipLaunchKernelGGL(k_bucket_apply, TG, dim3(WG), 0, st, bflag, count, btot, L.NB, nb, tiles, bstart, alist, slotof, mark,
                        start, misc);
-    hipLaunchKernelGGL(k_mark_compact, dim3(nblocks(L.NB)), dim3(WG), 0, st, mark, L.NB, blist, misc);
-    if (n > 0) hipLaunchKernelGGL(k_bin_scatter, dim3(nblocks(n)), dim3(WG), 0, st, n, key, slot, start, perm, inv);
+    const long long nmark = nblocks(L.NB);
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)(nmark + (n > 0 ? nblocks(n) : 0))), dim3(WG), 0, st, n, key, slot, start,
+                       perm, inv, nmark, mark, L.NB, blist, misc);
     if (stable && n > 0)
         hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm, inv);
     HIPCHK(hipGetLastError());

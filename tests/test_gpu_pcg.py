@@ -9,6 +9,10 @@ and solved directly), itself pinned by tests/test_oracle_pcg.py.  Here:
   default tolerance 1e-6: kinematic fields within 1e-5 and damage within
   2e-5 (the solve's tolerance plus fp32 assembly rounding);
 * the solver's reported residual meets the tolerance, in a few iterations;
+* on the reference's step fixtures with interface particles (the
+  directional stress split) and with f_clamp, in fp64: every state field
+  and the diagnostics tau_last and J_last within 1e-9 of the oracle's
+  implicit step;
 * at the million-particle config1 scene: a uniform damage field under a
   uniform drive solves to the closed form (d + c D) / (1 + c (1 + D))
   (K 1 = 0), a size-independent check of the assembly, the fold, the
@@ -21,7 +25,7 @@ import pytest
 import torch
 
 from oracle import cpu_step as O
-from tests.golden_cases import rel_err
+from tests.golden_cases import attach_boundaries, case_config, case_tensors, load_npz, rel_err
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -116,3 +120,30 @@ def test_pcg_uniform_field_closed_form_at_config1_scale():
     assert err <= 2e-6 * want, err
     assert rel <= 1e-6
     assert bool(torch.equal(sim.D_hist, torch.full_like(sim.D_hist, D0)))
+
+
+@pytest.mark.parametrize("name", ["directional32", "fclamp_damp_every2_on"])
+def test_pcg_step_on_interface_and_fclamp_fixtures(name):
+    from peel_mpm_amd import AnisoMPM, halfspace_collider, sphere_collider
+    rec = load_npz(f"step_{name}.npz")
+    cfg, time0, _ = case_config(rec)
+    ins = {k: v.double() if v.is_floating_point() else v for k, v in case_tensors(rec).items()}
+    gpu = AnisoMPM(device=DEV, dtype=torch.float64, damage_solver="pcg", pcg_tol=1e-12, pcg_max_iter=200,
+                   exact_sigma_eig=True, **cfg)
+    for k, v in ins.items():
+        setattr(gpu, k, v.to(DEV))
+    gpu.n = gpu.x.shape[0]
+    ora = O.OracleMPM(dtype=torch.float64, damage_solver="implicit", **cfg).load(ins)
+    gpu._step = ora._step = int(rec["in__step"])
+    attach_boundaries(gpu, rec, sphere_collider, halfspace_collider, device=DEV)
+    attach_boundaries(ora, rec, O.SphereCollider, O.HalfspaceCollider)
+    assert bool(ins["interface_mask"].any()) or cfg["f_clamp"] is not None
+    for s in range(3):                       # damage_every 2: solve, plain, solve
+        t = time0 + s * cfg["dt"]
+        print(f"pcg fixture {name} step {s}", flush=True)
+        gpu.substep(t)
+        ora.substep(t)
+        errs = {k: rel_err(getattr(gpu, k), getattr(ora, k)) for k in STATE + ("tau_last", "J_last")}
+        print(name, s, {k: f"{e:.1e}" for k, e in errs.items()}, flush=True)
+        for k, e in errs.items():
+            assert e <= 1e-9, (name, s, k, e)
