@@ -251,8 +251,9 @@ int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
 const char *pmpm_last_error(void);
 const char *pmpm_version(void);
 /* sizeof of pmpm_collider, pmpm_step_params, pmpm_particles_in,
- * pmpm_particles_out, pmpm_frame_in, pmpm_frame_obs (in that order) for
- * binding-layout checks; returns the number of sizes written (at most n). */
+ * pmpm_particles_out, pmpm_frame_in, pmpm_frame_obs, pmpm_pcg64 (in that
+ * order) for binding-layout checks; returns the number of sizes written (at
+ * most n). */
 int pmpm_abi_sizes(size_t *sizes, int32_t n);
 
 /* ---- after the step: the renderer's inputs -------------------------------- */
@@ -310,6 +311,56 @@ typedef struct {
 size_t pmpm_observe_workspace_bytes(void);
 int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame_in *in,
                        const pmpm_frame_obs *obs, void *ws, size_t ws_bytes, void *stream);
+
+/* ---- scene builders (setup) ------------------------------------------------ */
+/* The reference's particle-set constructors, bit-exact with their NumPy /
+ * SciPy results.  Jitter draws come from NumPy's default_rng bit generator
+ * (PCG64); the caller seeds it (NumPy's SeedSequence) and passes the seeded
+ * state: the 128-bit LCG state and increment. */
+typedef struct {
+    uint64_t state_hi, state_lo, inc_hi, inc_lo;
+} pmpm_pcg64;
+
+/* Bytes of scratch the builders' stable compaction needs for n candidates. */
+size_t pmpm_compact_workspace_bytes(int64_t n_candidates);
+
+/* ball_particles (anisompm_py/src/peel_test.py:128-137): the points of the
+ * lattice g x g x g (g: (n,) fp64, device; candidates in meshgrid "ij" C
+ * order) with sqrt(x^2 + y^2 + z^2) < R, each moved by (u - 0.5) js (u: the
+ * generator's draws, three per kept point in order) and by center (host,
+ * 3), rounded to fp32 into out ((cap, 3), device).  *count (host) gets the
+ * number of points (also when it exceeds cap, which fails with
+ * PMPM_ESIZE).  Synchronises stream once (the count); the points are
+ * written on stream. */
+int pmpm_ball_particles(const double *g, int64_t n, double R, double js, const double *center, const pmpm_pcg64 *rng,
+                        float *out, int64_t cap, int64_t *count, void *ws, size_t ws_bytes, void *stream);
+
+/* grid_block (anisompm_py/src/test_block.py:41-47): out ((nx ny nz, 3) fp64,
+ * device) = the "ij" meshgrid of xs (nx), ys (ny), zs (nz) (device, fp64 --
+ * the per-axis linspaces), raveled in C order. */
+int pmpm_grid_block(const double *xs, const double *ys, const double *zs, int64_t nx, int64_t ny, int64_t nz,
+                    double *out, void *stream);
+
+/* fill_interior (anisompm_py/src/orange_setup.py:65-92), in two calls.
+ * Points w ((n,3), device, w_dtype PMPM_F32 or PMPM_F64); arithmetic in
+ * ct_dtype, the type NumPy's promotion gives the caller's points, centre and
+ * scale (fp32 only when all three are fp32).
+ * pmpm_fill_bounds: sim = (w - center) * scale + 0.5; writes lo (host, 3) =
+ * the minimum of sim per axis and dims (host, 3) = max floor((sim - lo) / dx)
+ * + 3 per axis.  Workspace: pmpm_fill_workspace_bytes(NULL).
+ * pmpm_fill_interior: marks the points' voxels in a dims grid, closes it
+ * (dilation, then erosion, by the 6-neighbour cross, 0 outside the array),
+ * fills its holes (background not connected through background to the
+ * array's faces), and writes one point per solid voxel in C order, lo +
+ * (ijk + 0.5) dx, moved by (u - 0.5) js when rng is not NULL, rounded to fp32
+ * into out ((cap, 3), device).  *count as for pmpm_ball_particles.
+ * Workspace: pmpm_fill_workspace_bytes(dims).  Both synchronise stream. */
+size_t pmpm_fill_workspace_bytes(const int64_t *dims);
+int pmpm_fill_bounds(int32_t w_dtype, int32_t ct_dtype, const void *w, int64_t n, const double *center, double scale,
+                     double dx, double *lo, int64_t *dims, void *ws, size_t ws_bytes, void *stream);
+int pmpm_fill_interior(int32_t w_dtype, int32_t ct_dtype, const void *w, int64_t n, const double *center, double scale,
+                       double dx, const double *lo, const int64_t *dims, double js, const pmpm_pcg64 *rng, float *out,
+                       int64_t cap, int64_t *count, void *ws, size_t ws_bytes, void *stream);
 
 #ifdef __cplusplus
 }

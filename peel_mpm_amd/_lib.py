@@ -24,7 +24,12 @@ EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_
            "pmpm_sym_eig", "pmpm_last_error", "pmpm_version", "pmpm_abi_sizes", "pmpm_timing", "pmpm_timing_read",
            "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info",
            "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry", "pmpm_observe_workspace_bytes",
-           "pmpm_observe_frame")
+           "pmpm_observe_frame", "pmpm_compact_workspace_bytes", "pmpm_ball_particles", "pmpm_grid_block",
+           "pmpm_fill_workspace_bytes", "pmpm_fill_bounds", "pmpm_fill_interior")
+
+
+class Pcg64(C.Structure):
+    _fields_ = [("state_hi", C.c_uint64), ("state_lo", C.c_uint64), ("inc_hi", C.c_uint64), ("inc_lo", C.c_uint64)]
 
 
 class Collider(C.Structure):
@@ -109,10 +114,20 @@ def lib():
     L.pmpm_observe_frame.argtypes = [i32, i64, i32, C.POINTER(FrameIn), C.POINTER(FrameObs), vp, sz, vp]
     L.pmpm_gaussian_carry.argtypes = [i32, i64, i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, C.POINTER(dbl), dbl, vp, vp,
                                       vp, vp]
-    sizes = (sz * 6)()
-    L.pmpm_abi_sizes(sizes, 6)
+    L.pmpm_compact_workspace_bytes.restype = sz
+    L.pmpm_compact_workspace_bytes.argtypes = [i64]
+    L.pmpm_ball_particles.argtypes = [vp, i64, dbl, dbl, C.POINTER(dbl), C.POINTER(Pcg64), vp, i64, C.POINTER(i64), vp, sz,
+                                      vp]
+    L.pmpm_grid_block.argtypes = [vp, vp, vp, i64, i64, i64, vp, vp]
+    L.pmpm_fill_workspace_bytes.restype = sz
+    L.pmpm_fill_workspace_bytes.argtypes = [C.POINTER(i64)]
+    L.pmpm_fill_bounds.argtypes = [i32, i32, vp, i64, C.POINTER(dbl), dbl, dbl, C.POINTER(dbl), C.POINTER(i64), vp, sz, vp]
+    L.pmpm_fill_interior.argtypes = [i32, i32, vp, i64, C.POINTER(dbl), dbl, dbl, C.POINTER(dbl), C.POINTER(i64), dbl,
+                                     C.POINTER(Pcg64), vp, i64, C.POINTER(i64), vp, sz, vp]
+    sizes = (sz * 7)()
+    L.pmpm_abi_sizes(sizes, 7)
     mine = (C.sizeof(Collider), C.sizeof(StepParams), C.sizeof(ParticlesIn), C.sizeof(ParticlesOut), C.sizeof(FrameIn),
-            C.sizeof(FrameObs))
+            C.sizeof(FrameObs), C.sizeof(Pcg64))
     if tuple(sizes) != mine:
         raise ImportError(f"libpmpm ABI mismatch: library struct sizes {tuple(sizes)} != binding {mine}")
     _lib = L

@@ -1921,6 +1921,7 @@ __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p
 #include "pmpm_adjoint.h"
 #include "pmpm_carry.h"
 #include "pmpm_observe.h"
+#include "pmpm_scene.h"
 
 // ------------------------------------------------------------------ test kernels
 template <typename T>
@@ -2249,10 +2250,10 @@ const char* pmpm_last_error(void) { return g_err; }
 const char* pmpm_version(void) { return "pmpm 0.1 gfx950"; }
 
 int pmpm_abi_sizes(size_t* sizes, int32_t n) {
-    const size_t all[6] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
-                           sizeof(pmpm_particles_out), sizeof(pmpm_frame_in), sizeof(pmpm_frame_obs)};
+    const size_t all[7] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
+                           sizeof(pmpm_particles_out), sizeof(pmpm_frame_in), sizeof(pmpm_frame_obs), sizeof(pmpm_pcg64)};
     int k = 0;
-    for (; k < n && k < 6; ++k) sizes[k] = all[k];
+    for (; k < n && k < 7; ++k) sizes[k] = all[k];
     return k;
 }
 
@@ -2508,6 +2509,86 @@ int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame
     hipStream_t st = (hipStream_t)stream;
     return dtype == PMPM_F64 ? launch_observe<double>(n, frame, in, obs, (double*)ws, st)
                              : launch_observe<float>(n, frame, in, obs, (double*)ws, st);
+}
+
+size_t pmpm_compact_workspace_bytes(int64_t n_candidates) { return n_candidates < 0 ? 0 : sc_bytes(n_candidates); }
+
+int pmpm_ball_particles(const double* g, int64_t n, double R, double js, const double* center, const pmpm_pcg64* rng,
+                        float* out, int64_t cap, int64_t* count, void* ws, size_t ws_bytes, void* stream) {
+    if (!g || !center || !rng || !out || !count || !ws) return fail(PMPM_EARG, "%snull argument", "");
+    if (n < 1 || n > 2000) return fail(PMPM_EARG, "%slattice size out of range: %lld", "", n);
+    if (ws_bytes < sc_bytes(n * n * n))
+        return fail(PMPM_ESIZE, "%scompaction workspace too small, need %lld bytes", "", (long long)sc_bytes(n * n * n));
+    return launch_ball(g, n, R, js, center, *rng, out, cap, (long long*)count, (char*)ws, (hipStream_t)stream);
+}
+
+int pmpm_grid_block(const double* xs, const double* ys, const double* zs, int64_t nx, int64_t ny, int64_t nz, double* out,
+                    void* stream) {
+    if (!xs || !ys || !zs || !out) return fail(PMPM_EARG, "%snull argument", "");
+    if (nx < 0 || ny < 0 || nz < 0 || (nx > 0 && ny > 0 && nz > (1LL << 40) / nx / ny))
+        return fail(PMPM_EARG, "%sbad block size", "");
+    const long long N = nx * ny * nz;
+    if (N == 0) return PMPM_OK;
+    hipLaunchKernelGGL(k_grid_block, dim3((unsigned)((N + WG - 1) / WG)), dim3(WG), 0, (hipStream_t)stream, xs, ys, zs, nx,
+                       ny, nz, out);
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
+}
+
+size_t pmpm_fill_workspace_bytes(const int64_t* dims) {
+    const size_t bounds = al(3 * FILL_PARTS * sizeof(double)) + al(3 * FILL_PARTS * sizeof(long long));
+    if (!dims) return bounds;
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1) return 0;
+    return std::max(bounds, fill_bytes(dims[0] * dims[1] * dims[2]));
+}
+
+static int fill_check(int32_t w_dtype, int32_t ct_dtype, const void* w, int64_t n, const double* center, double scale,
+                      double dx) {
+    if (w_dtype != PMPM_F32 && w_dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown point dtype %lld", "", w_dtype);
+    if (ct_dtype != PMPM_F32 && ct_dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown compute dtype %lld", "", ct_dtype);
+    if (w_dtype == PMPM_F64 && ct_dtype == PMPM_F32) return fail(PMPM_EARG, "%sfp64 points need fp64 arithmetic", "");
+    if (!w || !center) return fail(PMPM_EARG, "%snull argument", "");
+    if (n < 1) return fail(PMPM_EARG, "%sno points: %lld", "", n);
+    if (!(dx > 0.0) || !(scale == scale)) return fail(PMPM_EARG, "%svoxel size must be positive", "");
+    return PMPM_OK;
+}
+
+int pmpm_fill_bounds(int32_t w_dtype, int32_t ct_dtype, const void* w, int64_t n, const double* center, double scale,
+                     double dx, double* lo, int64_t* dims, void* ws, size_t ws_bytes, void* stream) {
+    int rc = fill_check(w_dtype, ct_dtype, w, n, center, scale, dx);
+    if (rc) return rc;
+    if (!lo || !dims || !ws) return fail(PMPM_EARG, "%snull argument", "");
+    if (ws_bytes < pmpm_fill_workspace_bytes(nullptr))
+        return fail(PMPM_ESIZE, "%sfill workspace too small, need %lld bytes", "", (long long)pmpm_fill_workspace_bytes(nullptr));
+    hipStream_t st = (hipStream_t)stream;
+    long long* D = (long long*)dims;
+    if (w_dtype == PMPM_F64) return launch_fill_bounds<double, double>((const double*)w, n, center, scale, dx, lo, D, (char*)ws, st);
+    if (ct_dtype == PMPM_F64) return launch_fill_bounds<float, double>((const float*)w, n, center, scale, dx, lo, D, (char*)ws, st);
+    return launch_fill_bounds<float, float>((const float*)w, n, center, scale, dx, lo, D, (char*)ws, st);
+}
+
+int pmpm_fill_interior(int32_t w_dtype, int32_t ct_dtype, const void* w, int64_t n, const double* center, double scale,
+                       double dx, const double* lo, const int64_t* dims, double js, const pmpm_pcg64* rng, float* out,
+                       int64_t cap, int64_t* count, void* ws, size_t ws_bytes, void* stream) {
+    int rc = fill_check(w_dtype, ct_dtype, w, n, center, scale, dx);
+    if (rc) return rc;
+    if (!lo || !dims || !out || !count || !ws) return fail(PMPM_EARG, "%snull argument", "");
+    if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[0] > (1LL << 20) || dims[1] > (1LL << 20) || dims[2] > (1LL << 20) ||
+        dims[0] * dims[1] > (1LL << 34) / dims[2])
+        return fail(PMPM_EARG, "%svoxel grid out of range", "");
+    if (ws_bytes < pmpm_fill_workspace_bytes(dims))
+        return fail(PMPM_ESIZE, "%sfill workspace too small, need %lld bytes", "", (long long)pmpm_fill_workspace_bytes(dims));
+    hipStream_t st = (hipStream_t)stream;
+    const long long* D = (const long long*)dims;
+    long long* cnt = (long long*)count;
+    if (w_dtype == PMPM_F64)
+        return launch_fill_interior<double, double>((const double*)w, n, center, scale, dx, lo, D, js, rng, out, cap, cnt,
+                                                    (char*)ws, st);
+    if (ct_dtype == PMPM_F64)
+        return launch_fill_interior<float, double>((const float*)w, n, center, scale, dx, lo, D, js, rng, out, cap, cnt,
+                                                   (char*)ws, st);
+    return launch_fill_interior<float, float>((const float*)w, n, center, scale, dx, lo, D, js, rng, out, cap, cnt, (char*)ws,
+                                              st);
 }
 
 }  // extern "C"

@@ -1,0 +1,94 @@
+"""Device scene builders (peel_mpm_amd.builders) against the reference's
+fixtures and the oracle (needs a GPU).  Every case is bit-exact: the
+builders' jitter is NumPy's default_rng stream and their arithmetic is
+NumPy's, in NumPy's order and types.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import scene_builders as O
+from tests.golden_cases import load_npz
+from tests.test_oracle_builders import cloud
+
+pytestmark = pytest.mark.gpu
+
+
+def test_ball_matches_the_reference_peel_runs():
+    from peel_mpm_amd.builders import ball_particles
+    z = load_npz("peel_ball.npz")
+    print("ball: reference peel runs", z["x_start"].shape, flush=True)
+    got = ball_particles(z["center"], float(z["R"]), float(z["spacing"]), float(z["jitter"]), int(z["seed"]))
+    assert got.dtype == np.float32 and np.array_equal(got, z["x_start"])
+
+
+@pytest.mark.parametrize("center,R,spacing,jitter,seed", [
+    ((0.3, 0.6, 0.45), 0.1, 0.004, 0.5, 7),
+    ((0.5, 0.5, 0.5), 0.28, 1 / 128, 0.3, 0),
+    (np.array([0.5, 0.5, 0.5], np.float32), 0.45, 1 / 200, 0.3, 11),   # 3 M points
+])
+def test_ball_matches_the_oracle(center, R, spacing, jitter, seed):
+    from peel_mpm_amd.builders import ball_particles
+    print(f"ball R={R} spacing={spacing} seed={seed}", flush=True)
+    got = ball_particles(center, R, spacing, jitter, seed)
+    want = O.ball_particles(center, R, spacing, jitter, seed)
+    print("points", len(want), flush=True)
+    assert got.shape == want.shape and np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("lo,hi,n", [((0.3, 0.4, 0.3), (0.7, 0.5, 0.7), (40, 10, 40)),
+                                     ((0.1, 0.2, 0.3), (0.5, 0.4, 0.9), (4, 3, 5)),
+                                     ((0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (128, 96, 64))])
+def test_grid_block_matches_the_oracle(lo, hi, n):
+    from peel_mpm_amd.builders import grid_block
+    print(f"grid_block {n}", flush=True)
+    got = grid_block(lo, hi, n)
+    want = O.grid_block(lo, hi, n)
+    assert got.dtype == np.float64 and np.array_equal(got, want)
+
+
+def orange_like(n, seed=0):
+    """A thick shell of fp32 points (a peel) around a sparser core."""
+    g = np.random.default_rng(seed)
+    v = g.normal(size=(n, 3))
+    r = np.where(g.random(n) < 0.7, 0.9 + 0.1 * g.random(n), g.random(n) ** (1 / 3) * 0.9)
+    return (0.05 * r[:, None] * v / np.linalg.norm(v, axis=1, keepdims=True) + [0.01, 0.2, -0.03]).astype(np.float32)
+
+
+CASES = [
+    # the orange pipeline's types: fp32 points, fp32 centre, fp32 scale (fp32 arithmetic)
+    ("orange fp32", lambda: orange_like(200_000), "f32", np.float32(12.4), 1 / 256, True, 0),
+    # a fp64 centre promotes the sim frame to fp64
+    ("orange fp64 centre", lambda: orange_like(200_000, 1), "f64", 12.4, 1 / 256, True, 3),
+    ("blob no jitter", lambda: cloud("blob", 30_000, 2), "f32", np.float32(1.0), 0.02, False, 0),
+    ("shell + anchor", lambda: np.concatenate([cloud("shell", 50_000, 4), np.full((1, 3), -0.5, np.float32)]), "f32",
+     np.float32(1.0), 0.02, True, 9),
+    ("orange 2M", lambda: orange_like(2_000_000, 5), "f32", np.float32(12.4), 1 / 512, True, 1),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_fill_interior_matches_the_oracle(case):
+    from peel_mpm_amd.builders import fill_interior
+    name, make, cdt, scale, vox, jitter, seed = case
+    pts = make()
+    lo, hi = pts.min(0), pts.max(0)
+    center = (np.float32(0.5) * (lo + hi)) if cdt == "f32" else (0.5 * (lo.astype(np.float64) + hi))
+    print(f"fill_interior {name}: {len(pts)} points", flush=True)
+    got = fill_interior(pts, center, scale, vox, jitter=jitter, seed=seed)
+    want = O.fill_interior(pts, center, scale, vox, jitter=jitter, seed=seed)
+    print("particles", len(want[0]), flush=True)
+    assert got[2] == want[2]
+    assert got[0].shape == want[0].shape and np.array_equal(got[0], want[0])
+    assert got[1].dtype == np.float32 and np.array_equal(got[1], want[1])
+
+
+def test_builders_return_device_tensors():
+    import torch
+    from peel_mpm_amd.builders import ball_particles, fill_interior
+    t = ball_particles((0.5, 0.5, 0.5), 0.1, 0.01, as_tensor=True)
+    assert isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float32
+    x, v, dx = fill_interior(torch.from_numpy(cloud("blob", 5000)).cuda(), np.zeros(3, np.float32), np.float32(1.0),
+                             0.05, as_tensor=True)
+    assert x.is_cuda and v.is_cuda and len(x) == len(v) > 0
