@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
                     [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
-                    [--no-config2] [--no-carry] [--no-tune]
+                    [--no-config2] [--no-carry] [--no-builders] [--no-tune]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
@@ -252,6 +252,68 @@ def time_carry(spec, splats_per_particle, launches):
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
 
 
+def orange_cloud(n, seed=0):
+    """A synthetic stand-in for the orange's Gaussian centres (fp32): 70 %
+    in a thick peel shell, the rest through the core, 5 cm across."""
+    import numpy as np
+    g = np.random.default_rng(seed)
+    v = g.normal(size=(n, 3))
+    r = np.where(g.random(n) < 0.7, 0.9 + 0.1 * g.random(n), g.random(n) ** (1 / 3) * 0.9)
+    return (0.05 * r[:, None] * v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+
+
+def time_builders(repeats, cpu):
+    """The scene builders on the GPU (peel_mpm_amd.builders) at the
+    reference's scales: fill_interior on 3.93 M fp32 points placed as the
+    orange pipeline places them (fp32 centre and scale, 62 % of the box,
+    voxels of half a 64-grid cell), and the peel ball (R 0.28, spacing
+    1/128).  Each call includes its host synchronisations.  With ``cpu``,
+    the oracle (the reference's NumPy / SciPy code) times the same calls on
+    this host as the baseline."""
+    import numpy as np
+    import torch
+    from peel_mpm_amd.builders import ball_particles, fill_interior
+    pts = orange_cloud(3_930_000)
+    lo, hi = pts.min(0), pts.max(0)
+    center = np.float32(0.5) * (lo + hi)
+    scale = np.float32(0.62) / (hi - lo).max()
+    vox = (1.0 / 64) / 2.0
+    wd = torch.from_numpy(pts).cuda()
+    res = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            r = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / repeats, r
+
+    t, (x, _, _) = timed(lambda: fill_interior(wd, center, scale, vox, as_tensor=True))
+    res["fill_interior"] = {"workload": f"{len(pts)} fp32 points (synthetic orange-sized cloud), voxel 1/128",
+                            "points_in": len(pts), "particles_out": int(x.shape[0]), "ms_per_call": t * 1e3,
+                            "value": len(pts) / t, "unit": "input points/s"}
+    t, xb = timed(lambda: ball_particles((0.5, 0.5, 0.5), 0.28, 1.0 / 128, as_tensor=True))
+    res["ball_particles"] = {"workload": "peel ball: R 0.28, spacing 1/128, jitter 0.3", "particles_out": int(xb.shape[0]),
+                             "ms_per_call": t * 1e3, "value": xb.shape[0] / t, "unit": "particles/s"}
+    if cpu:
+        from oracle import scene_builders as O
+        t0 = time.perf_counter()
+        O.fill_interior(pts, center, scale, vox)
+        tc = time.perf_counter() - t0
+        res["fill_interior"]["cpu_baseline"] = {"ms_per_call": tc * 1e3, "value": len(pts) / tc, "unit": "input points/s",
+                                                "cores": 1, "kind": "port",
+                                                "sample": "oracle (the reference's NumPy + SciPy fill_interior), one call"}
+        t0 = time.perf_counter()
+        O.ball_particles((0.5, 0.5, 0.5), 0.28, 1.0 / 128)
+        tc = time.perf_counter() - t0
+        res["ball_particles"]["cpu_baseline"] = {"ms_per_call": tc * 1e3, "value": xb.shape[0] / tc,
+                                                 "unit": "particles/s", "cores": 1, "kind": "port",
+                                                 "sample": "oracle (the reference's NumPy ball_particles), one call"}
+    return res
+
+
 OBS_B_PER_PARTICLE = 42     # phi, phi_shear, d 12, x 12, region + grip 2 in; phi_pp, psh_pp 16 read and written
 
 
@@ -343,6 +405,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry and peel-observable legs")
+    ap.add_argument("--no-builders", action="store_true", help="skip the scene-builder leg")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--launch-config", default=None, metavar="JSON",
@@ -389,6 +452,8 @@ def main():
     if not args.no_carry:
         out["gaussian_carry"] = time_carry(spec, 4, 20)
         out["peel_observables"] = time_observe(spec, 50)
+    if not args.no_builders:
+        out["scene_builders"] = time_builders(3, cpu=not args.no_cpu_baseline)
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")
