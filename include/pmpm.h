@@ -312,6 +312,17 @@ size_t pmpm_observe_workspace_bytes(void);
 int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame_in *in,
                        const pmpm_frame_obs *obs, void *ws, size_t ws_bytes, void *stream);
 
+/* ---- Option A directional damage (the variational pair) --------------------- */
+/* energy() and directional_pk1() (anisompm_py/src/anisompm.py:129-187): per
+ * particle the energy density Psi (psi, (N,)) and its first Piola-Kirchhoff
+ * stress P = dPsi/dF (P, (N,3,3)) for F (N,3,3), reference interface normals
+ * n0 (N,3) and damage d (N,); mu, lam, gamma_I, gamma_II per particle from
+ * the *_arr arrays (N,) or, when an array is NULL, the scalar.  Either output
+ * may be NULL.  One kernel, enqueued on stream. */
+int pmpm_directional_pk1(int32_t dtype, int64_t n, const void *F, const void *n0, const void *d, const void *mu_arr,
+                         double mu, const void *lam_arr, double lam, const void *gamma_I_arr, double gamma_I,
+                         const void *gamma_II_arr, double gamma_II, void *P, void *psi, void *stream);
+
 /* ---- scene builders (setup) ------------------------------------------------ */
 /* The reference's particle-set constructors, bit-exact with their NumPy /
  * SciPy results.  Jitter draws come from NumPy's default_rng bit generator

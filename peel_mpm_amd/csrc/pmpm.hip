@@ -1922,6 +1922,7 @@ __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p
 #include "pmpm_carry.h"
 #include "pmpm_observe.h"
 #include "pmpm_scene.h"
+#include "pmpm_variational.h"
 
 // ------------------------------------------------------------------ test kernels
 template <typename T>
@@ -2509,6 +2510,20 @@ int pmpm_observe_frame(int32_t dtype, int64_t n, int32_t frame, const pmpm_frame
     hipStream_t st = (hipStream_t)stream;
     return dtype == PMPM_F64 ? launch_observe<double>(n, frame, in, obs, (double*)ws, st)
                              : launch_observe<float>(n, frame, in, obs, (double*)ws, st);
+}
+
+int pmpm_directional_pk1(int32_t dtype, int64_t n, const void* F, const void* n0, const void* d, const void* mu_arr,
+                         double mu, const void* lam_arr, double lam, const void* gamma_I_arr, double gamma_I,
+                         const void* gamma_II_arr, double gamma_II, void* P, void* psi, void* stream) {
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
+    if (n < 0) return fail(PMPM_EARG, "%sbad size %lld", "", n);
+    if (n == 0) return PMPM_OK;
+    if (!F || !n0 || !d || (!P && !psi)) return fail(PMPM_EARG, "%snull argument", "");
+    hipStream_t st = (hipStream_t)stream;
+    return dtype == PMPM_F64 ? launch_option_a<double>(n, F, n0, d, mu_arr, mu, lam_arr, lam, gamma_I_arr, gamma_I,
+                                                       gamma_II_arr, gamma_II, P, psi, st)
+                             : launch_option_a<float>(n, F, n0, d, mu_arr, mu, lam_arr, lam, gamma_I_arr, gamma_I,
+                                                      gamma_II_arr, gamma_II, P, psi, st);
 }
 
 size_t pmpm_compact_workspace_bytes(int64_t n_candidates) { return n_candidates < 0 ? 0 : sc_bytes(n_candidates); }
