@@ -34,6 +34,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 #include "../../include/pmpm.h"
@@ -1806,9 +1808,12 @@ __global__ void __launch_bounds__(WG) k_pcg_update(StepArgs<T> A) {
 
 // One work-group: the partial dots in a fixed order, then the CG scalars.
 enum { PCG_RED_INIT = 0, PCG_RED_ALPHA = 1, PCG_RED_BETA = 2 };
+// hflag (host-mapped, NULL under stream capture): thread 0 publishes
+// (seq << 32) | (done ? 0xFFFFFFFF : iterations) with one 8-byte system-scope
+// store, for the host's wait between iteration chunks.
 template <int MODE>
 __global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ part, int nparts, double* __restrict__ sc,
-                                                   double tol, int max_iter) {
+                                                   double tol, int max_iter, unsigned long long* hflag, unsigned seq) {
     if (MODE != PCG_RED_INIT && sc[PCG_DONE] != 0.0) return;
     constexpr int NCH = MODE == PCG_RED_INIT ? 3 : (MODE == PCG_RED_ALPHA ? 1 : 2);
     __shared__ double s[NCH][WG];
@@ -1852,6 +1857,54 @@ __global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ pa
         sc[PCG_ITER] = it;
         sc[PCG_REL] = rel;
         sc[PCG_DONE] = (rel <= tol || it >= (double)max_iter) ? 1.0 : 0.0;
+    }
+    if (hflag) {
+        const unsigned lo = sc[PCG_DONE] != 0.0 ? 0xFFFFFFFFu : (unsigned)sc[PCG_ITER];
+        __hip_atomic_store(hflag, ((unsigned long long)seq << 32) | lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+// Per host thread and device, 64 bytes of pinned host memory the solve's
+// reductions publish their state to (allocated once, held until the process
+// ends), and the sequence number of the solve using it.
+struct PcgFlag {
+    int dev;
+    unsigned long long *host, *dptr;
+    unsigned seq;
+};
+thread_local std::vector<PcgFlag> t_pcg_flags;
+
+PcgFlag* pcg_flag() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    for (auto& f : t_pcg_flags)
+        if (f.dev == dev) return &f;
+    void* h = nullptr;
+    void* d = nullptr;
+    if (hipHostMalloc(&h, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) return nullptr;
+    if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) return nullptr;
+    memset(h, 0, 64);
+    t_pcg_flags.push_back({dev, (unsigned long long*)h, (unsigned long long*)d, 0});
+    return &t_pcg_flags.back();
+}
+
+// Wait for the published state of solve seq to show `issued` iterations
+// done or convergence: 1 converged, 0 not yet, -1 not seen within a second
+// (the caller then reads the device flag the synchronous way).
+int pcg_wait(const PcgFlag& f, int issued) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spin = 1;; ++spin) {
+        const unsigned long long v = __atomic_load_n(f.host, __ATOMIC_ACQUIRE);
+        if ((unsigned)(v >> 32) == f.seq) {
+            const unsigned lo = (unsigned)v;
+            if (lo == 0xFFFFFFFFu) return 1;
+            if (lo >= (unsigned)issued) return 0;
+        }
+        if ((spin & 255) == 0) {
+            const auto dt = std::chrono::steady_clock::now() - t0;
+            if (dt > std::chrono::seconds(1)) return -1;
+            if (dt > std::chrono::microseconds(200)) std::this_thread::yield();
+        }
     }
 }
 
@@ -2142,17 +2195,26 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY_DIRECT>), GG, GB, 0, st, A);
         hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
-        hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts, A.pscal,
-                           A.pcg_tol, A.pcg_max);
         // Iterations go out in chunks of 1, 2, 4, ... with a read of the done
         // flag between chunks, so a solve that converges in a few iterations
         // does not pay for pcg_max_iter launches (nor, converging in one, for
-        // a second iteration's launches).  Under stream capture no host read
-        // is possible: all pcg_max_iter iterations are enqueued and the ones
-        // after convergence exit at their first load.
+        // a second iteration's launches).  The reductions publish the solve's
+        // state to pinned host memory, which the host polls: no copy and no
+        // stream synchronisation between chunks.  Under stream capture no
+        // host read is possible: all pcg_max_iter iterations are enqueued and
+        // the ones after convergence exit at their first load.
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         HIPCHK(hipStreamIsCapturing(st, &cap));
         const bool capturing = cap != hipStreamCaptureStatusNone;
+        PcgFlag* pf = capturing ? nullptr : pcg_flag();
+        unsigned long long* hflag = nullptr;
+        unsigned seq = 0;
+        if (pf) {
+            seq = ++pf->seq;
+            hflag = pf->dptr;
+        }
+        hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts, A.pscal,
+                           A.pcg_tol, A.pcg_max, hflag, seq);
         int issued = 0, chunk = 1;
         while (issued < p->pcg_max_iter) {
             const int k_end = capturing ? p->pcg_max_iter : std::min(p->pcg_max_iter, issued + chunk);
@@ -2160,16 +2222,21 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
                 hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
                 hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
                 hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts,
-                                   A.pscal, A.pcg_tol, A.pcg_max);
+                                   A.pscal, A.pcg_tol, A.pcg_max, hflag, seq);
                 hipLaunchKernelGGL(k_pcg_update<T>, GG, GB, 0, st, A);
                 hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_BETA>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts,
-                                   A.pscal, A.pcg_tol, A.pcg_max);
+                                   A.pscal, A.pcg_tol, A.pcg_max, hflag, seq);
             }
             if (capturing || issued >= p->pcg_max_iter) break;
-            double flag = 0.0;
-            HIPCHK(hipMemcpyAsync(&flag, A.pscal + PCG_DONE, sizeof flag, hipMemcpyDeviceToHost, st));
-            HIPCHK(hipStreamSynchronize(st));
-            if (flag != 0.0) break;
+            HIPCHK(hipGetLastError());
+            int done = pf ? pcg_wait(*pf, issued) : -1;
+            if (done < 0) {
+                double flag = 0.0;
+                HIPCHK(hipMemcpyAsync(&flag, A.pscal + PCG_DONE, sizeof flag, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipStreamSynchronize(st));
+                done = flag != 0.0;
+            }
+            if (done) break;
             chunk *= 2;
         }
         hipLaunchKernelGGL(k_pcg_interp<T>, PG, PB, 0, st, A);
