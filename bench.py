@@ -69,7 +69,7 @@ def time_gpu(spec, steps, warmup, tune=True):
     sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
     L = _lib.lib()
     t_tune = time.perf_counter()
-    launch = autotune.tune_solver(sim, steps=5) if tune else autotune.get_config()
+    launch = autotune.tune_solver(sim, steps=10) if tune else autotune.get_config()
     t_tune = time.perf_counter() - t_tune
     for s in range(warmup):
         sim.substep(s * spec["dt"])
@@ -121,7 +121,7 @@ def time_batched(n_scenes, steps, warmup):
         b.add_scene(scenes.tensors(sp, "cuda:0"),
                     [VelocityBC(torch.from_numpy(m).to("cuda:0"), tuple(v)) for m, v in sp["bc"]])
     t_tune = time.perf_counter()
-    launch = autotune.tune_batched(b, steps=4)         # before the graph is captured
+    launch = autotune.tune_batched(b, steps=6)         # before the graph is captured
     t_tune = time.perf_counter() - t_tune
     for s in range(warmup):
         b.substep(s * 1e-4)

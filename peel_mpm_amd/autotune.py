@@ -77,7 +77,7 @@ def _time_steps(step, restore, steps, device):
     return (time.perf_counter() - t0) / steps
 
 
-def tune(step, restore, device, space=None, steps=3, repeats=2, sweeps=1, log=None):
+def tune(step, restore, device, space=None, steps=10, repeats=3, sweeps=1, log=None):
     """Coordinate descent over ``space`` (default SPACE).  ``step()`` advances
     the workload one step, ``restore()`` puts its state back.  Returns
     (best config, {config tuple: seconds per step}) and leaves the best
@@ -114,7 +114,7 @@ def _solver_key(sim, kind):
             getattr(sim, "damage_solver", "explicit"))
 
 
-def tune_solver(sim, steps=3, space=None, log=None, use_cache=True):
+def tune_solver(sim, steps=10, space=None, log=None, use_cache=True):
     """Tune the launch shape on an AnisoMPM's current scene (its state is
     restored afterwards).  Returns the chosen config."""
     key = _solver_key(sim, "plain")
@@ -139,7 +139,7 @@ def tune_solver(sim, steps=3, space=None, log=None, use_cache=True):
     return best
 
 
-def tune_batched(b, steps=3, space=None, log=None, use_cache=True):
+def tune_batched(b, steps=10, space=None, log=None, use_cache=True):
     """Tune the launch shape on a BatchedAnisoMPM before its graph is
     captured (tuning runs the step eagerly; the graph is captured with the
     chosen shape at the next substep).  Its state is restored afterwards."""
