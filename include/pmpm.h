@@ -341,8 +341,8 @@ size_t pmpm_compact_workspace_bytes(int64_t n_candidates);
  * generator's draws, three per kept point in order) and by center (host,
  * 3), rounded to fp32 into out ((cap, 3), device).  *count (host) gets the
  * number of points (also when it exceeds cap, which fails with
- * PMPM_ESIZE).  Synchronises stream once (the count); the points are
- * written on stream. */
+ * PMPM_ESIZE).  n^3 must stay below 2^31 (n <= 1290).  Synchronises stream
+ * once (the count); the points are written on stream. */
 int pmpm_ball_particles(const double *g, int64_t n, double R, double js, const double *center, const pmpm_pcg64 *rng,
                         float *out, int64_t cap, int64_t *count, void *ws, size_t ws_bytes, void *stream);
 
@@ -364,7 +364,8 @@ int pmpm_grid_block(const double *xs, const double *ys, const double *zs, int64_
  * fills its holes (background not connected through background to the
  * array's faces), and writes one point per solid voxel in C order, lo +
  * (ijk + 0.5) dx, moved by (u - 0.5) js when rng is not NULL, rounded to fp32
- * into out ((cap, 3), device).  *count as for pmpm_ball_particles.
+ * into out ((cap, 3), device).  *count as for pmpm_ball_particles.  The
+ * voxel grid must hold fewer than 2^31 voxels.
  * Workspace: pmpm_fill_workspace_bytes(dims).  Both synchronise stream. */
 size_t pmpm_fill_workspace_bytes(const int64_t *dims);
 int pmpm_fill_bounds(int32_t w_dtype, int32_t ct_dtype, const void *w, int64_t n, const double *center, double scale,

@@ -2598,7 +2598,8 @@ size_t pmpm_compact_workspace_bytes(int64_t n_candidates) { return n_candidates 
 int pmpm_ball_particles(const double* g, int64_t n, double R, double js, const double* center, const pmpm_pcg64* rng,
                         float* out, int64_t cap, int64_t* count, void* ws, size_t ws_bytes, void* stream) {
     if (!g || !center || !rng || !out || !count || !ws) return fail(PMPM_EARG, "%snull argument", "");
-    if (n < 1 || n > 2000) return fail(PMPM_EARG, "%slattice size out of range: %lld", "", n);
+    // the compaction's offsets are 32-bit: fewer than 2^31 candidates
+    if (n < 1 || n > 1290) return fail(PMPM_EARG, "%slattice size out of range: %lld", "", n);
     if (ws_bytes < sc_bytes(n * n * n))
         return fail(PMPM_ESIZE, "%scompaction workspace too small, need %lld bytes", "", (long long)sc_bytes(n * n * n));
     return launch_ball(g, n, R, js, center, *rng, out, cap, (long long*)count, (char*)ws, (hipStream_t)stream);
@@ -2655,8 +2656,9 @@ int pmpm_fill_interior(int32_t w_dtype, int32_t ct_dtype, const void* w, int64_t
     int rc = fill_check(w_dtype, ct_dtype, w, n, center, scale, dx);
     if (rc) return rc;
     if (!lo || !dims || !out || !count || !ws) return fail(PMPM_EARG, "%snull argument", "");
+    // the compaction's offsets are 32-bit: fewer than 2^31 voxels
     if (dims[0] < 1 || dims[1] < 1 || dims[2] < 1 || dims[0] > (1LL << 20) || dims[1] > (1LL << 20) || dims[2] > (1LL << 20) ||
-        dims[0] * dims[1] > (1LL << 34) / dims[2])
+        dims[0] * dims[1] >= (1LL << 31) / dims[2])
         return fail(PMPM_EARG, "%svoxel grid out of range", "");
     if (ws_bytes < pmpm_fill_workspace_bytes(dims))
         return fail(PMPM_ESIZE, "%sfill workspace too small, need %lld bytes", "", (long long)pmpm_fill_workspace_bytes(dims));

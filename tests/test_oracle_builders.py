@@ -209,6 +209,8 @@ def test_builder_entry_points_reject_bad_arguments_before_any_gpu_work():
                                  1 << 20, None) != 0
     assert L.pmpm_ball_particles(C.c_void_p(16), 10, 0.3, 0.01, c3, C.byref(rng), C.c_void_p(16), 1000, C.byref(cnt), ws,
                                  1, None) != 0
+    assert L.pmpm_ball_particles(C.c_void_p(16), 1291, 0.3, 0.01, c3, C.byref(rng), C.c_void_p(16), 1000, C.byref(cnt),
+                                 ws, 1 << 40, None) != 0
     assert L.pmpm_grid_block(None, None, None, 2, 2, 2, None, None) != 0
     lo = (C.c_double * 3)()
     dims = (C.c_int64 * 3)()
@@ -219,6 +221,9 @@ def test_builder_entry_points_reject_bad_arguments_before_any_gpu_work():
                               None) != 0
     assert L.pmpm_fill_bounds(_lib.PMPM_F32, _lib.PMPM_F32, C.c_void_p(16), 10, c3, 1.0, 0.0, lo, dims, ws, nb,
                               None) != 0
+    huge = (C.c_int64 * 3)(1 << 11, 1 << 10, 1 << 10)
+    assert L.pmpm_fill_interior(_lib.PMPM_F32, _lib.PMPM_F32, C.c_void_p(16), 10, c3, 1.0, 0.1, lo, huge, 0.06, None,
+                                C.c_void_p(16), 100, C.byref(cnt), ws, 1 << 40, None) != 0
     bad = (C.c_int64 * 3)(4, 0, 4)
     assert L.pmpm_fill_workspace_bytes(bad) == 0
     assert L.pmpm_fill_interior(_lib.PMPM_F32, _lib.PMPM_F32, C.c_void_p(16), 10, c3, 1.0, 0.1, lo, bad, 0.06, None,
