@@ -323,6 +323,16 @@ int pmpm_directional_pk1(int32_t dtype, int64_t n, const void *F, const void *n0
                          double mu, const void *lam_arr, double lam, const void *gamma_I_arr, double gamma_I,
                          const void *gamma_II_arr, double gamma_II, void *P, void *psi, void *stream);
 
+/* directional_kirchhoff (anisompm_py/src/anisompm.py:190-227) as the step
+ * uses it for interface particles (:693-703), through the step's own code:
+ * tau = Q tau_c Q + gI P tau_c P + gII (P tau_c Q + Q tau_c P), P = m m^T,
+ * Q = I - P, m = F n0 / (|F n0| + 1e-9), gI = gII = (1 - d)^2 (1 - k_r) + k_r
+ * (k_r per particle from res_arr (N,), or res when it is NULL); with
+ * tension_gate, gI = 1 where m^T tau_c m <= 0.  tau_c, F, tau (N,3,3); d
+ * (N,); n0 (N,3).  One kernel, enqueued on stream. */
+int pmpm_directional_kirchhoff(int32_t dtype, int64_t n, const void *tau_c, const void *F, const void *d, const void *n0,
+                               const void *res_arr, double res, int32_t tension_gate, void *tau, void *stream);
+
 /* ---- scene builders (setup) ------------------------------------------------ */
 /* The reference's particle-set constructors, bit-exact with their NumPy /
  * SciPy results.  Jitter draws come from NumPy's default_rng bit generator

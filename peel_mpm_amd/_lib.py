@@ -25,7 +25,8 @@ EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_
            "pmpm_adjoint_workspace_bytes", "pmpm_substep_adjoint", "pmpm_damage_solve_info",
            "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry", "pmpm_observe_workspace_bytes",
            "pmpm_observe_frame", "pmpm_compact_workspace_bytes", "pmpm_ball_particles", "pmpm_grid_block",
-           "pmpm_fill_workspace_bytes", "pmpm_fill_bounds", "pmpm_fill_interior", "pmpm_directional_pk1")
+           "pmpm_fill_workspace_bytes", "pmpm_fill_bounds", "pmpm_fill_interior", "pmpm_directional_pk1",
+           "pmpm_directional_kirchhoff")
 
 
 class Pcg64(C.Structure):
@@ -115,6 +116,7 @@ def lib():
     L.pmpm_gaussian_carry.argtypes = [i32, i64, i64, vp, vp, vp, vp, vp, vp, vp, dbl, dbl, C.POINTER(dbl), dbl, vp, vp,
                                       vp, vp]
     L.pmpm_directional_pk1.argtypes = [i32, i64, vp, vp, vp, vp, dbl, vp, dbl, vp, dbl, vp, dbl, vp, vp, vp]
+    L.pmpm_directional_kirchhoff.argtypes = [i32, i64, vp, vp, vp, vp, vp, dbl, i32, vp, vp]
     L.pmpm_compact_workspace_bytes.restype = sz
     L.pmpm_compact_workspace_bytes.argtypes = [i64]
     L.pmpm_ball_particles.argtypes = [vp, i64, dbl, dbl, C.POINTER(dbl), C.POINTER(Pcg64), vp, i64, C.POINTER(i64), vp, sz,

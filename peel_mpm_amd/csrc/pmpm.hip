@@ -540,6 +540,57 @@ __device__ __forceinline__ void load_const_in(const StepArgs<T>& A, long long p,
     }
 }
 
+// Degraded Kirchhoff stress from the undegraded deviatoric and volumetric
+// parts: g(d) on the deviatoric part, on the volumetric part only in
+// tension (anisompm.py:687-691), and for interface particles the split by
+// the deformed normal F n0 (anisompm.py:190-227, :697-703).  gate: the
+// tension gate (the mode-I degradation only where the normal traction
+// pulls); the step always gates, as the reference's solver does.
+template <typename T>
+__device__ __forceinline__ M3<T> degrade(T res, const M3<T>& F, T d, T J, const M3<T>& tdev, T tvol, bool iface,
+                                         const T n0[3], bool gate = true) {
+    const T om = T(1) - d;
+    const T g = om * om * (T(1) - res) + res;
+    const T vdeg = J >= T(1) ? g : T(1);
+    M3<T> tau;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) tau.m[i][j] = g * tdev.m[i][j] + (i == j ? vdeg * tvol : T(0));
+    if (iface) {
+        // projection split by the deformed normal (anisompm.py:190-227, :697-703)
+        M3<T> tc = tdev;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tc.m[i][i] += tvol;
+        T m[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = F.m[i][0] * n0[0] + F.m[i][1] * n0[1] + F.m[i][2] * n0[2];
+        const T mn = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) + T(1e-9);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) m[i] = m[i] / mn;
+        M3<T> P, Q;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                P.m[i][j] = m[i] * m[j];
+                Q.m[i][j] = (i == j ? T(1) : T(0)) - P.m[i][j];
+            }
+        M3<T> Pt = mul(P, tc), Qt = mul(Q, tc);
+        M3<T> tperp = mul(Pt, P), tpq = mul(Pt, Q), tqp = mul(Qt, P), tpar = mul(Qt, Q);
+        T tm[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) tm[i] = tc.m[i][0] * m[0] + tc.m[i][1] * m[1] + tc.m[i][2] * m[2];
+        const T snn = m[0] * tm[0] + m[1] * tm[1] + m[2] * tm[2];
+        const T gI = (snn > T(0) || !gate) ? g : T(1);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tau.m[i][j] = tpar.m[i][j] + gI * tperp.m[i][j] + g * (tpq.m[i][j] + tqp.m[i][j]);
+    }
+        return tau;
+}
+
 // Stress, damage and degradation of one particle (anisompm.py:614-706).
 // Returns the degraded Kirchhoff stress; F is replaced by its clamped value
 // when f_clamp is on; d becomes the post-update damage.  DRIVE: stop after
@@ -678,48 +729,7 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         A.od[p] = d;
     }
 
-    // degradation g(d), volumetric part only in tension (anisompm.py:687-691)
-    const T res = q.res;
-    const T om = T(1) - d;
-    const T g = om * om * (T(1) - res) + res;
-    const T vdeg = J >= T(1) ? g : T(1);
-    M3<T> tau;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) tau.m[i][j] = g * tdev.m[i][j] + (i == j ? vdeg * tvol : T(0));
-    if (iface) {
-        // projection split by the deformed normal (anisompm.py:190-227, :697-703)
-        M3<T> tc = tdev;
-#pragma unroll
-        for (int i = 0; i < 3; ++i) tc.m[i][i] += tvol;
-        T m[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) m[i] = F.m[i][0] * n0[0] + F.m[i][1] * n0[1] + F.m[i][2] * n0[2];
-        const T mn = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]) + T(1e-9);
-#pragma unroll
-        for (int i = 0; i < 3; ++i) m[i] = m[i] / mn;
-        M3<T> P, Q;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                P.m[i][j] = m[i] * m[j];
-                Q.m[i][j] = (i == j ? T(1) : T(0)) - P.m[i][j];
-            }
-        M3<T> Pt = mul(P, tc), Qt = mul(Q, tc);
-        M3<T> tperp = mul(Pt, P), tpq = mul(Pt, Q), tqp = mul(Qt, P), tpar = mul(Qt, Q);
-        T tm[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) tm[i] = tc.m[i][0] * m[0] + tc.m[i][1] * m[1] + tc.m[i][2] * m[2];
-        const T snn = m[0] * tm[0] + m[1] * tm[1] + m[2] * tm[2];
-        const T gI = snn > T(0) ? g : T(1);
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) tau.m[i][j] = tpar.m[i][j] + gI * tperp.m[i][j] + g * (tpq.m[i][j] + tqp.m[i][j]);
-    }
-    return tau;
+    return degrade<T>(q.res, F, d, J, tdev, tvol, iface, n0);
 }
 
 // Stress, damage and degradation per particle, in particle order (the
@@ -1977,6 +1987,20 @@ __global__ void __launch_bounds__(WG) k_zero_bytes(unsigned char* __restrict__ p
 #include "pmpm_scene.h"
 #include "pmpm_variational.h"
 
+// Option B on its own (the interface branch of the step's stress): tau =
+// the projection split of tau_c by m = F n0 / |F n0| with g(d) = (1 - d)^2
+// (1 - k_r) + k_r for both modes, through the step's own degrade().
+template <typename T>
+__global__ void __launch_bounds__(WG) k_dir_kirchhoff(long long n, const T* __restrict__ tau_c, const T* __restrict__ Fp,
+                                                      const T* __restrict__ dp, const T* __restrict__ n0p,
+                                                      const T* __restrict__ res_a, T res_s, int gate, T* __restrict__ out) {
+    const long long p = (long long)blockIdx.x * WG + threadIdx.x;
+    if (p >= n) return;
+    const T n0[3] = {n0p[3 * p], n0p[3 * p + 1], n0p[3 * p + 2]};
+    const M3<T> tc = load_m3(tau_c, p);
+    store_m3(out, p, degrade<T>(res_a ? res_a[p] : res_s, load_m3(Fp, p), dp[p], T(1), tc, T(0), true, n0, gate != 0));
+}
+
 // ------------------------------------------------------------------ test kernels
 template <typename T>
 __global__ void __launch_bounds__(WG) k_sym_eig(const T* __restrict__ A, long long n, T* __restrict__ ev, T* __restrict__ V) {
@@ -2591,6 +2615,24 @@ int pmpm_directional_pk1(int32_t dtype, int64_t n, const void* F, const void* n0
                                                        gamma_II_arr, gamma_II, P, psi, st)
                              : launch_option_a<float>(n, F, n0, d, mu_arr, mu, lam_arr, lam, gamma_I_arr, gamma_I,
                                                       gamma_II_arr, gamma_II, P, psi, st);
+}
+
+int pmpm_directional_kirchhoff(int32_t dtype, int64_t n, const void* tau_c, const void* F, const void* d, const void* n0,
+                               const void* res_arr, double res, int32_t tension_gate, void* tau, void* stream) {
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
+    if (n < 0) return fail(PMPM_EARG, "%sbad size %lld", "", n);
+    if (n == 0) return PMPM_OK;
+    if (!tau_c || !F || !d || !n0 || !tau) return fail(PMPM_EARG, "%snull argument", "");
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 G((unsigned)((n + WG - 1) / WG)), B(WG);
+    if (dtype == PMPM_F64)
+        hipLaunchKernelGGL(k_dir_kirchhoff<double>, G, B, 0, st, n, (const double*)tau_c, (const double*)F, (const double*)d,
+                           (const double*)n0, (const double*)res_arr, res, tension_gate, (double*)tau);
+    else
+        hipLaunchKernelGGL(k_dir_kirchhoff<float>, G, B, 0, st, n, (const float*)tau_c, (const float*)F, (const float*)d,
+                           (const float*)n0, (const float*)res_arr, (float)res, tension_gate, (float*)tau);
+    HIPCHK(hipGetLastError());
+    return PMPM_OK;
 }
 
 size_t pmpm_compact_workspace_bytes(int64_t n_candidates) { return n_candidates < 0 ? 0 : sc_bytes(n_candidates); }

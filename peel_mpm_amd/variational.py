@@ -1,5 +1,6 @@
-"""Option A directional damage on the GPU: the reference's variationally
-consistent energy / stress pair (``anisompm_py/src/anisompm.py:129-187``).
+"""Directional damage on the GPU: the reference's Option A, a variationally
+consistent energy / stress pair (``anisompm_py/src/anisompm.py:129-187``),
+and its Option B projection split (``:190-227``) as a standalone call.
 
     psi = energy(F, n0, d, mu, lam, gamma_I, gamma_II)           # (N,)
     P   = directional_pk1(F, n0, d, mu, lam, gamma_I, gamma_II)  # (N,3,3), dPsi/dF
@@ -12,7 +13,8 @@ computes both per particle.  The step keeps the reference's Option B
 (the projection split in ``k_const``), as the reference's solver does;
 this pair is the differentiable alternative the reference validates with
 its finite-difference gate (``1_fd_check.py``), which the tests repeat on
-the device.
+the device.  ``directional_kirchhoff`` runs the step's own Option B code on
+given stresses, for the reference's reconstruction and release gates.
 """
 from __future__ import annotations
 
@@ -78,6 +80,30 @@ def energy_and_pk1(F, n0, d, mu, lam, gamma_I, gamma_II):
     """Both from one launch: (psi (N,), P (N,3,3))."""
     P, psi = _run(F, n0, d, mu, lam, gamma_I, gamma_II, True, True)
     return psi, P
+
+
+def directional_kirchhoff(tau_c, F, d, n0, residual=1e-3, tension_gate=True):
+    """Option B, the projection split the step applies at interface
+    particles (anisompm.py:190-227 as called at :693-703): tau_c split by
+    the deformed normal, normal and shear blocks degraded by
+    g(d) = (1 - d)^2 (1 - k_r) + k_r, the normal one only in tension when
+    ``tension_gate``.  ``residual`` k_r: a scalar or (N,) tensor.  Runs the
+    step's own device code (``pmpm_directional_kirchhoff``)."""
+    if not (tau_c.is_cuda and F.is_cuda and d.is_cuda and n0.is_cuda):
+        raise ValueError("the directional split runs on the GPU only")
+    n = F.shape[0]
+    if tau_c.shape != (n, 3, 3) or F.shape != (n, 3, 3) or d.shape != (n,) or n0.shape != (n, 3):
+        raise ValueError("expected tau_c, F (N,3,3), d (N,), n0 (N,3)")
+    if not (tau_c.dtype == F.dtype == d.dtype == n0.dtype):
+        raise TypeError("tau_c, F, d and n0 must share one dtype")
+    tau_c, F, d, n0 = tau_c.contiguous(), F.contiguous(), d.contiguous(), n0.contiguous()
+    r, rptr, rs = _param(residual, n, F)
+    out = torch.empty_like(tau_c)
+    _lib.check(_lib.lib().pmpm_directional_kirchhoff(_lib.dtype_code(F.dtype), n, tau_c.data_ptr(), F.data_ptr(),
+                                                     d.data_ptr(), n0.data_ptr(), rptr, rs, int(bool(tension_gate)),
+                                                     out.data_ptr(), _lib.stream_ptr(F.device)),
+               "pmpm_directional_kirchhoff")
+    return out
 
 
 def kirchhoff(F, n0, d, mu, lam, gamma_I, gamma_II):

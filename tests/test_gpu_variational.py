@@ -8,7 +8,11 @@ restatement of the reference's energy / directional_pk1 (needs a GPU).
   device energy itself: its central differences match the device stress to
   1e-6 relative at d = 0, 0.5, 0.9;
 * per-particle parameters (N,) against scalars; stress-only and energy-only
-  launches against the joint one; tau = P F^T.
+  launches against the joint one; tau = P F^T;
+* Option B through the step's own device code (pmpm_directional_kirchhoff):
+  the reference's Gate 2 and Gate 3 (1_fd_check.py:79-120) against its
+  recorded run, and the oracle's split at random d, with and without the
+  tension gate, per-particle residuals.
 """
 from __future__ import annotations
 
@@ -17,7 +21,8 @@ import torch
 
 from oracle import variational as O
 from tests.golden_cases import rel_err
-from tests.test_oracle_variational import PARAMS, gate_inputs
+from tests.test_oracle_variational import (GATE_RES, PARAMS, RECORDED_KEPT, RECORDED_RECON, corotated, gate_inputs,
+                                           kept_fractions)
 
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
@@ -83,3 +88,36 @@ def test_per_particle_parameters_and_partial_outputs():
     assert torch.equal(V.directional_pk1(F, n0, d, mu, lam, 5.0, 3.0), P)
     tau = V.kirchhoff(F, n0, d, mu, lam, 5.0, 3.0)
     assert torch.allclose(tau, P @ F.transpose(1, 2), rtol=1e-13, atol=1e-13)
+
+
+def test_option_b_gates_on_the_step_code():
+    from peel_mpm_amd import variational as V
+    F, n0 = gate_inputs(64, seed=1)
+    tau_c = corotated(F)
+    d0 = torch.zeros(F.shape[0], dtype=torch.float64)
+    split = V.directional_kirchhoff(tau_c.to(DEV), F.to(DEV), d0.to(DEV), n0.to(DEV), residual=GATE_RES).cpu()
+    err = (split - tau_c).abs().max().item()
+    print(f"gate 2 reconstruction {err:.2e} (recorded {RECORDED_RECON:.2e})", flush=True)
+    assert err < 1e-9 and err < 100 * RECORDED_RECON
+    F, n0 = gate_inputs(64, seed=2)
+    tau_c = corotated(F)
+    d1 = torch.ones(F.shape[0], dtype=torch.float64)
+    tau1 = V.directional_kirchhoff(tau_c.to(DEV), F.to(DEV), d1.to(DEV), n0.to(DEV), residual=GATE_RES,
+                                   tension_gate=False).cpu()
+    kept = kept_fractions(tau_c, tau1, F, n0)
+    print(f"gate 3 kept {kept} (recorded {RECORDED_KEPT})", flush=True)
+    for got, want in zip(kept, RECORDED_KEPT):
+        assert abs(got - want) < 1e-9
+
+
+@pytest.mark.parametrize("gate", [True, False])
+def test_option_b_matches_the_oracle(gate):
+    from oracle.cpu_step import directional_tau
+    from peel_mpm_amd import variational as V
+    F, n0, d = inputs(20_000, 5, torch.float64)
+    tau_c = corotated(F)
+    res = 1e-3 + 0.01 * torch.rand(F.shape[0], dtype=torch.float64, generator=torch.Generator().manual_seed(6))
+    got = V.directional_kirchhoff(tau_c.to(DEV), F.to(DEV), d.to(DEV), n0.to(DEV), residual=res.to(DEV),
+                                  tension_gate=gate)
+    want = directional_tau(tau_c, F, d, n0, res, tension_gate=gate)
+    assert rel_err(got, want) <= 1e-13

@@ -8,7 +8,11 @@ argument checks of its device entry point (no GPU work).
 * closed forms: uniaxial stretch along n0 (matrix + mode I, mode I off in
   compression) and simple shear across n0 (mode II = gamma_II / 2 g^2);
 * zero energy and stress at F = I and at a pure rotation; frame invariance
-  Psi(QF) = Psi(F), P(QF) = Q P(F).
+  Psi(QF) = Psi(F), P(QF) = Q P(F);
+* Option B (the step's projection split, oracle/cpu_step.directional_tau)
+  on the reference's Gate 2 and Gate 3 samples (1_fd_check.py:79-120):
+  the d=0 reconstruction and the d=1 kept fractions against the values its
+  recorded run wrote to out/validate.jsonl.
 """
 from __future__ import annotations
 
@@ -21,6 +25,11 @@ from oracle import variational as O
 
 PARAMS = dict(mu=10.0, lam=20.0, gamma_I=5.0, gamma_II=3.0)
 RECORDED_FD_REL = 2.964987481764463e-10      # optionA_max_rel_err in the reference's out/validate.jsonl
+# the reference's recorded Option B gates (out/validate.jsonl): d=0 reconstruction error, and at d=1 the
+# kept fractions of the normal traction, in-plane stress and interfacial shear, residual k_r = 0.005
+RECORDED_RECON = 1.4210854715202004e-14
+RECORDED_KEPT = (0.00499999999993469, 0.9999999994358679, 0.005000003081451347)
+GATE_RES = 0.005
 
 
 def gate_inputs(n=64, seed=0):
@@ -116,3 +125,42 @@ def test_entry_point_rejects_bad_arguments_before_any_gpu_work():
     assert b"null" in L.pmpm_last_error()
     assert L.pmpm_directional_pk1(_lib.PMPM_F64, 0, None, None, None, None, 1.0, None, 1.0, None, 1.0, None, 1.0,
                                   None, None, None) == 0
+
+
+def corotated(F, mu=10.0, lam=20.0):
+    """The plain corotated Kirchhoff stress of the gates (anisompm.py:228-241)."""
+    U, S, Vh = torch.linalg.svd(F)
+    R = U @ Vh
+    J = torch.linalg.det(F)
+    I3 = torch.eye(3, dtype=F.dtype).expand_as(F)
+    return 2.0 * mu * (F - R) @ F.transpose(1, 2) + (lam * (J - 1.0) * J).view(-1, 1, 1) * I3
+
+
+def kept_fractions(tau_c, tau1, F, n0):
+    """release_check's measures (anisompm.py:279-291): normal, in-plane, shear."""
+    I3 = torch.eye(3, dtype=F.dtype).expand_as(F)
+    m = torch.einsum('pij,pj->pi', F, n0)
+    m = m / (m.norm(dim=1, keepdim=True) + 1e-9)
+    Pm = torch.einsum('pi,pj->pij', m, m)
+    Qm = I3 - Pm
+    nrm = lambda t: t.reshape(F.shape[0], -1).norm(dim=1)
+    tnc = torch.einsum('pi,pij,pj->p', m, tau_c, m).abs()
+    tn1 = torch.einsum('pi,pij,pj->p', m, tau1, m).abs()
+    ip = (nrm(Qm @ tau1 @ Qm) / (nrm(Qm @ tau_c @ Qm) + 1e-9)).min().item()
+    sh = (nrm(Pm @ tau1 @ Qm + Qm @ tau1 @ Pm) / (nrm(Pm @ tau_c @ Qm + Qm @ tau_c @ Pm) + 1e-9)).max().item()
+    return (tn1 / (tnc + 1e-9)).max().item(), ip, sh
+
+
+def test_option_b_gates_against_the_recorded_run():
+    from oracle.cpu_step import directional_tau
+    F, n0 = gate_inputs(64, seed=1)                    # Gate 2 samples (1_fd_check.py:81-88)
+    tau_c = corotated(F)
+    d0 = torch.zeros(F.shape[0], dtype=torch.float64)
+    err = (directional_tau(tau_c, F, d0, n0, 0.0) - tau_c).abs().max().item()
+    assert err < 1e-9 and err < 100 * RECORDED_RECON, err
+    F, n0 = gate_inputs(64, seed=2)                    # Gate 3 samples (1_fd_check.py:103-111)
+    tau_c = corotated(F)
+    d1 = torch.ones(F.shape[0], dtype=torch.float64)
+    kept = kept_fractions(tau_c, directional_tau(tau_c, F, d1, n0, GATE_RES, tension_gate=False), F, n0)
+    for got, want in zip(kept, RECORDED_KEPT):
+        assert abs(got - want) < 1e-9, (kept, RECORDED_KEPT)
