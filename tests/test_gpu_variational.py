@@ -121,3 +121,33 @@ def test_option_b_matches_the_oracle(gate):
                                   tension_gate=gate)
     want = directional_tau(tau_c, F, d, n0, res, tension_gate=gate)
     assert rel_err(got, want) <= 1e-13
+
+
+def test_reference_gate_functions_over_the_device_split():
+    """reconstruction_check / release_check with the reference's signatures,
+    on CPU tensors as 1_fd_check.py passes them."""
+    from peel_mpm_amd import variational as V
+    F, n0 = gate_inputs(64, seed=1)
+    U, S, Vh = torch.linalg.svd(F)
+    err = V.reconstruction_check(F, U @ Vh, torch.linalg.det(F), n0, mu=10.0, lam=20.0, verbose=False)
+    assert err < 1e-9
+    F, n0 = gate_inputs(64, seed=2)
+    U, S, Vh = torch.linalg.svd(F)
+    ok, nk, ik, sk = V.release_check(F, U @ Vh, torch.linalg.det(F), n0, mu=10.0, lam=20.0, res=GATE_RES,
+                                     verbose=False)
+    assert ok
+    for got, want in zip((nk, ik, sk), RECORDED_KEPT):
+        assert abs(got - want) < 1e-9
+
+
+def test_cpu_inputs_as_the_reference_gate_passes_them():
+    """1_fd_check.py's Gate 1 calls energy / directional_pk1 on CPU fp64
+    tensors: they compute on the GPU and return CPU tensors, equal to the
+    CUDA-path results."""
+    from peel_mpm_amd import variational as V
+    F, n0 = gate_inputs(64, seed=0)
+    d = torch.full((F.shape[0],), 0.5, dtype=torch.float64)
+    psi, P = V.energy(F, n0, d, **PARAMS), V.directional_pk1(F, n0, d, **PARAMS)
+    assert psi.device.type == "cpu" and P.device.type == "cpu"
+    psi_g, P_g = V.energy_and_pk1(F.to(DEV), n0.to(DEV), d.to(DEV), **PARAMS)
+    assert torch.equal(psi, psi_g.cpu()) and torch.equal(P, P_g.cpu())

@@ -128,7 +128,7 @@ def test_entry_point_rejects_bad_arguments_before_any_gpu_work():
 
 
 def corotated(F, mu=10.0, lam=20.0):
-    """The plain corotated Kirchhoff stress of the gates (anisompm.py:228-241)."""
+    """The plain corotated Kirchhoff stress of the gates (anisompm.py:230-241)."""
     U, S, Vh = torch.linalg.svd(F)
     R = U @ Vh
     J = torch.linalg.det(F)
