@@ -314,6 +314,52 @@ def time_builders(repeats, cpu):
     return res
 
 
+OPTA_B_PER_PARTICLE = 92   # F 36, n0 12, d 4 in; P 36, psi 4 out (fp32)
+
+
+def time_option_a(n, launches, cpu):
+    """Option A (peel_mpm_amd.variational: energy and dPsi/dF in one
+    launch) on n particles of random near-identity F (I + 0.15 N(0,1), det
+    > 0.2), random unit normals and damage: per-particle rate and the HBM
+    roofline at 92 B per particle.  With ``cpu``, the oracle (the
+    reference's torch functions) on a 100 K sample on this host."""
+    import torch
+    from peel_mpm_amd import variational as V
+    g = torch.Generator(device="cuda:0").manual_seed(0)
+    F = torch.eye(3, device="cuda:0").repeat(n, 1, 1) + 0.15 * torch.randn(n, 3, 3, generator=g, device="cuda:0")
+    F[torch.linalg.det(F) <= 0.2] = torch.eye(3, device="cuda:0")
+    n0 = torch.randn(n, 3, generator=g, device="cuda:0")
+    n0 = n0 / n0.norm(dim=1, keepdim=True)
+    d = torch.rand(n, generator=g, device="cuda:0")
+    prm = dict(mu=10.0, lam=20.0, gamma_I=5.0, gamma_II=3.0)
+    for _ in range(2):
+        V.energy_and_pk1(F, n0, d, **prm)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(launches):
+        V.energy_and_pk1(F, n0, d, **prm)
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / launches
+    gbs = OPTA_B_PER_PARTICLE * n / per / 1e9
+    out = {"workload": f"{n} particles, random near-identity F, fp32; per call through the Python wrapper "
+                       "(output allocation included)", "value": n / per, "unit": "particles/s",
+           "ms_per_launch": per * 1e3,
+           "roofline": {"bound": "hbm", "algorithmic_bytes": OPTA_B_PER_PARTICLE * n, "achieved": gbs,
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}}
+    if cpu:
+        from oracle import variational as O
+        m = 100_000
+        Fc, nc, dc = F[:m].double().cpu(), n0[:m].double().cpu(), d[:m].double().cpu()
+        t0 = time.perf_counter()
+        O.energy(Fc, nc, dc, **prm)
+        O.directional_pk1(Fc, nc, dc, **prm)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": m / tc, "unit": "particles/s", "cores": torch.get_num_threads(), "kind": "port",
+                               "sample": f"oracle (the reference's torch energy + directional_pk1, fp64) on {m} of "
+                                         f"the particles, {tc:.2f} s"}
+    return out
+
+
 OBS_B_PER_PARTICLE = 42     # phi, phi_shear, d 12, x 12, region + grip 2 in; phi_pp, psh_pp 16 read and written
 
 
@@ -405,7 +451,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry and peel-observable legs")
-    ap.add_argument("--no-builders", action="store_true", help="skip the scene-builder leg")
+    ap.add_argument("--no-builders", action="store_true", help="skip the scene-builder and Option A legs")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--launch-config", default=None, metavar="JSON",
@@ -454,6 +500,7 @@ def main():
         out["peel_observables"] = time_observe(spec, 50)
     if not args.no_builders:
         out["scene_builders"] = time_builders(3, cpu=not args.no_cpu_baseline)
+        out["option_a"] = time_option_a(984_064, 20, cpu=not args.no_cpu_baseline)
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")
