@@ -92,3 +92,35 @@ def test_builders_return_device_tensors():
     x, v, dx = fill_interior(torch.from_numpy(cloud("blob", 5000)).cuda(), np.zeros(3, np.float32), np.float32(1.0),
                              0.05, as_tensor=True)
     assert x.is_cuda and v.is_cuda and len(x) == len(v) > 0
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_builder_parameters(seed):
+    """Random centres, radii, pitches, jitters and seeds for the ball, and
+    random clouds, frames, voxel sizes and input types for fill_interior:
+    bit-exact against the oracle."""
+    from peel_mpm_amd.builders import ball_particles, fill_interior
+    g = np.random.default_rng(100 + seed)
+    R = float(g.uniform(0.02, 0.3))
+    spacing = float(R / g.uniform(4.0, 40.0))
+    center = g.uniform(0.2, 0.8, 3)
+    if seed % 2:
+        center = center.astype(np.float32)
+    jitter, rs = float(g.uniform(0.0, 1.0)), int(g.integers(0, 2 ** 31))
+    print(f"random ball R={R:.4f} spacing={spacing:.5f} jitter={jitter:.2f} seed={rs}", flush=True)
+    assert np.array_equal(ball_particles(center, R, spacing, jitter, rs), O.ball_particles(center, R, spacing, jitter, rs))
+    n = int(g.integers(1, 60_000))
+    pts = (g.uniform(-1, 1) + g.uniform(0.01, 0.5) * g.normal(size=(n, 3)))
+    pts = pts.astype(np.float32 if seed % 3 else np.float64)
+    lo, hi = pts.min(0), pts.max(0)
+    ext = float((hi - lo).max()) or 1.0
+    if seed % 2:
+        center, scale = (pts.dtype.type(0.5) * (lo + hi)), np.float32(0.6) / np.float32(ext)
+    else:
+        center, scale = 0.5 * (lo.astype(np.float64) + hi), 0.6 / ext
+    vox = float(g.uniform(1 / 256, 1 / 32))
+    jit = bool(seed % 4 != 0)
+    print(f"random fill n={n} dtype={pts.dtype} vox={vox:.5f} jitter={jit}", flush=True)
+    got = fill_interior(pts, center, scale, vox, jitter=jit, seed=rs)
+    want = O.fill_interior(pts, center, scale, vox, jitter=jit, seed=rs)
+    assert got[2] == want[2] and np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
