@@ -108,6 +108,9 @@ def fill_interior(world_xyz, grid_center, scale_to_sim, vox_size, jitter=True, s
     w = torch.as_tensor(world_xyz)
     if w.dtype not in (torch.float32, torch.float64) or w.dim() != 2 or w.shape[1] != 3:
         raise ValueError("world_xyz must be (N, 3) fp32 or fp64")
+    if w.shape[0] == 0:
+        # orange_setup.py:69 takes the cloud's min and max, which NumPy refuses on no points
+        raise ValueError("world_xyz holds no points: an empty cloud has no bounds")
     center_np = np.asarray(grid_center)
     ct = np.result_type(np.float32 if w.dtype == torch.float32 else np.float64, center_np.dtype, scale_to_sim, 0.5)
     if ct not in (np.float32, np.float64):

@@ -56,6 +56,41 @@ def test_batched_matches_plain_scene_by_scene():
     assert float(batch.scene(2, "d").abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_ragged_batch_with_empty_scenes(use_graph):
+    """Empty scenes first, between and last in the batch: the others match
+    the plain step scene by scene bit for bit, and the empty ones stay
+    empty."""
+    from peel_mpm_amd import AnisoMPM, VelocityBC, scenes
+    from peel_mpm_amd.batched import BatchedAnisoMPM
+    specs = small_scenes(2)
+    print(f"ragged batch use_graph={use_graph}", flush=True)
+    b = BatchedAnisoMPM(n_grid=specs[0]["n_grid"], dt=specs[0]["dt"], gravity=specs[0]["gravity"], device=DEV,
+                        use_graph=use_graph)
+    empty = {k: v[:0] for k, v in scenes.tensors(specs[0], DEV).items()}
+    order = [None, 0, None, 1, None]
+    for i in order:
+        if i is None:
+            b.add_scene(empty)
+        else:
+            bcs = [VelocityBC(torch.from_numpy(m).to(DEV), tuple(v)) for m, v in specs[i]["bc"]]
+            b.add_scene(scenes.tensors(specs[i], DEV), bcs)
+    plain = [scenes.make_solver(AnisoMPM, sp, device=DEV) for sp in specs]
+    for s in range(5):
+        t = s * specs[0]["dt"]
+        b.substep(t)
+        for p in plain:
+            p.substep(t)
+    torch.cuda.synchronize()
+    assert b.n == sum(p.n for p in plain)
+    for slot, i in enumerate(order):
+        for k in KEYS:
+            if i is None:
+                assert b.scene(slot, k).shape[0] == 0
+            else:
+                assert torch.equal(b.scene(slot, k), getattr(plain[i], k)), (slot, k)
+
+
 def test_graph_replays_match_eager():
     specs = small_scenes(3)
     eager = build_batched(specs)

@@ -124,3 +124,21 @@ def test_random_builder_parameters(seed):
     got = fill_interior(pts, center, scale, vox, jitter=jit, seed=rs)
     want = O.fill_interior(pts, center, scale, vox, jitter=jit, seed=rs)
     assert got[2] == want[2] and np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+def test_degenerate_builder_inputs():
+    """A ball smaller than the lattice pitch (no point, or only the centre
+    point), a one-point cloud (a single voxel does not survive the closing),
+    a one-cell block: as the oracle."""
+    from peel_mpm_amd.builders import ball_particles, fill_interior, grid_block
+    for R in (0.0, 0.001, 0.005):
+        got, want = ball_particles((0.5, 0.5, 0.5), R, 0.01, 0.3, 0), O.ball_particles((0.5, 0.5, 0.5), R, 0.01, 0.3, 0)
+        print(f"ball R={R}: {len(want)} points", flush=True)
+        assert got.shape == want.shape and np.array_equal(got, want)
+    one = np.array([[0.1, 0.2, 0.3]], np.float32)
+    for jit in (True, False):
+        got = fill_interior(one, np.zeros(3, np.float32), np.float32(1.0), 0.05, jitter=jit)
+        want = O.fill_interior(one, np.zeros(3, np.float32), np.float32(1.0), 0.05, jitter=jit)
+        assert got[2] == want[2] and got[0].shape == want[0].shape == (0, 3) and got[1].shape == (0,)
+    got, want = grid_block((0.1, 0.2, 0.3), (0.2, 0.4, 0.5), (1, 1, 1)), O.grid_block((0.1, 0.2, 0.3), (0.2, 0.4, 0.5), (1, 1, 1))
+    assert np.array_equal(got, want)

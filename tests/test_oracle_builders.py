@@ -229,3 +229,22 @@ def test_builder_entry_points_reject_bad_arguments_before_any_gpu_work():
     assert L.pmpm_fill_interior(_lib.PMPM_F32, _lib.PMPM_F32, C.c_void_p(16), 10, c3, 1.0, 0.1, lo, bad, 0.06, None,
                                 C.c_void_p(16), 100, C.byref(cnt), ws, nb, None) != 0
     assert b"voxel grid" in L.pmpm_last_error()
+
+
+def test_empty_cloud_is_refused_as_the_reference_refuses_it():
+    """orange_setup.py:69 takes min and max of the cloud; NumPy raises
+    ValueError on no points, and so does fill_interior, before any GPU work;
+    the C entry point refuses n = 0 too."""
+    from peel_mpm_amd import _lib
+    from peel_mpm_amd.builders import fill_interior
+    empty = np.zeros((0, 3), np.float32)
+    with pytest.raises(ValueError):
+        O.fill_interior(empty, np.zeros(3, np.float32), np.float32(1.0), 0.05)
+    with pytest.raises(ValueError, match="no points"):
+        fill_interior(empty, np.zeros(3, np.float32), np.float32(1.0), 0.05)
+    L = _lib.lib()
+    lo, dims = (C.c_double * 3)(), (C.c_int64 * 3)()
+    c3 = (C.c_double * 3)(0.5, 0.5, 0.5)
+    assert L.pmpm_fill_bounds(_lib.PMPM_F32, _lib.PMPM_F32, C.c_void_p(16), 0, c3, 1.0, 0.1, lo, dims, C.c_void_p(16),
+                              L.pmpm_fill_workspace_bytes(None), None) != 0
+    assert b"no points" in L.pmpm_last_error()
