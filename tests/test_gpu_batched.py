@@ -15,12 +15,12 @@ DEV = "cuda:0"
 KEYS = ("x", "v", "F", "C", "d", "lap", "D_hist")
 
 
-def small_scenes(k=4):
+def small_scenes(k=4, dtype=torch.float32):
     from peel_mpm_amd import scenes
     specs = []
     for s in range(k):
         sp = scenes.slab_block(n_grid=32, cells=(6 + 2 * s, 4, 8 - s), origin=(8, 8 + s, 9), ppc=8, seed=s,
-                               E=1e5 * (1 + s), sigma_c=100.0 * (1 + s))
+                               E=1e5 * (1 + s), sigma_c=100.0 * (1 + s), dtype=dtype)
         specs.append(sp)
     return specs
 
@@ -28,7 +28,8 @@ def small_scenes(k=4):
 def build_batched(specs, **kw):
     from peel_mpm_amd import VelocityBC, scenes
     from peel_mpm_amd.batched import BatchedAnisoMPM
-    b = BatchedAnisoMPM(n_grid=specs[0]["n_grid"], dt=specs[0]["dt"], gravity=specs[0]["gravity"], device=DEV, **kw)
+    b = BatchedAnisoMPM(n_grid=specs[0]["n_grid"], dt=specs[0]["dt"], gravity=specs[0]["gravity"], device=DEV,
+                        dtype=specs[0]["dtype"], **kw)
     for sp in specs:
         t = scenes.tensors(sp, DEV)
         bcs = [VelocityBC(torch.from_numpy(m).to(DEV), tuple(v)) for m, v in sp["bc"]]
@@ -36,9 +37,11 @@ def build_batched(specs, **kw):
     return b
 
 
-def test_batched_matches_plain_scene_by_scene():
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_batched_matches_plain_scene_by_scene(dtype):
     from peel_mpm_amd import AnisoMPM, scenes
-    specs = small_scenes()
+    specs = small_scenes(dtype=dtype)
+    print(f"batched vs plain {dtype}", flush=True)
     batch = build_batched(specs)
     plain = [scenes.make_solver(AnisoMPM, sp, device=DEV) for sp in specs]
     # one scene without damage-allowed particles: its damage must stay untouched
@@ -91,8 +94,10 @@ def test_ragged_batch_with_empty_scenes(use_graph):
                 assert torch.equal(b.scene(slot, k), getattr(plain[i], k)), (slot, k)
 
 
-def test_graph_replays_match_eager():
-    specs = small_scenes(3)
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_graph_replays_match_eager(dtype):
+    specs = small_scenes(3, dtype)
+    print(f"graph vs eager {dtype}", flush=True)
     eager = build_batched(specs)
     graph = build_batched(specs, use_graph=True)
     for s in range(8):

@@ -59,17 +59,18 @@ def check(obs, oracle, frame_list, state):
         assert np.allclose(got, exp, rtol=1e-6, equal_nan=True), (f, got, exp)
 
 
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("onset_at", [6, FRAMES + 3])
-def test_observer_matches_oracle_on_frame_sequences(onset_at):
+def test_observer_matches_oracle_on_frame_sequences(onset_at, dtype):
     from peel_mpm_amd.observe import PeelObserver
     n = 50000
     interface, flesh, peel, cap = masks(n, seed=4)
-    fs = frames(n, seed=5, onset_at=onset_at)
-    state = State(n, torch.float32)
+    fs = [tuple(t.to(dtype) for t in f) for f in frames(n, seed=5, onset_at=onset_at)]
+    state = State(n, dtype)
     state.set(*(t.to(DEV) for t in fs[0]))
     obs = PeelObserver(state, interface, flesh, peel, cap, frames=FRAMES)
-    oracle = ObserverOracle(n, interface, flesh, peel, cap)
-    print(f"observer frames onset_at={onset_at} n={n}", flush=True)
+    oracle = ObserverOracle(n, interface, flesh, peel, cap, dtype=dtype)
+    print(f"observer frames onset_at={onset_at} n={n} dtype={dtype}", flush=True)
     for f, (phi, psh, d, x, tau, J) in enumerate(fs):
         state.set(phi.to(DEV), psh.to(DEV), d.to(DEV), x.to(DEV), tau.to(DEV), J.to(DEV))
         obs.observe(f)
