@@ -51,18 +51,31 @@ def n_cells(n_grid: int) -> int:
     return 64 * nb * nb * nb
 
 
-def bin_particles(x: torch.Tensor, n_grid: int, grid_lim: float = 1.0, stable: bool = True):
+def bin_workspace(n: int, n_grid: int, dtype=torch.float32, device="cuda:0"):
+    """An initialised workspace for bin_particles of up to n particles."""
+    L = _lib.lib()
+    code = _lib.dtype_code(dtype)
+    nbytes = L.pmpm_workspace_bytes(n, n_grid, 1, code)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    _lib.check(L.pmpm_workspace_init(ws.data_ptr(), nbytes, n, n_grid, 1, code, _lib.stream_ptr(torch.device(device))),
+               "pmpm_workspace_init")
+    return ws
+
+
+def bin_particles(x: torch.Tensor, n_grid: int, grid_lim: float = 1.0, stable: bool = True, ws=None):
     """Counting sort of particles by cell, cells numbered bucket-major (4x4x4
     blocks).  Returns (key, perm, cell_start) int32 device tensors; with
-    stable=True perm is the stable argsort of key."""
+    stable=True perm is the stable argsort of key.  ``ws`` (from
+    bin_workspace, for this n) is reused across calls as the step reuses
+    its own."""
     x = x.contiguous()
     n, dev = x.shape[0], x.device
     L = _lib.lib()
     code = _lib.dtype_code(x.dtype)
-    nbytes = L.pmpm_workspace_bytes(n, n_grid, 1, code)
-    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    if ws is None:
+        ws = bin_workspace(n, n_grid, x.dtype, dev)
+    nbytes = ws.numel()
     st = _lib.stream_ptr(dev)
-    _lib.check(L.pmpm_workspace_init(ws.data_ptr(), nbytes, n, n_grid, 1, code, st), "pmpm_workspace_init")
     key = torch.empty(n, device=dev, dtype=torch.int32)
     perm = torch.empty(n, device=dev, dtype=torch.int32)
     start = torch.empty(n_cells(n_grid) + 1, device=dev, dtype=torch.int32)

@@ -107,6 +107,30 @@ def test_binning_stable_sort_bit_exact(ng, n, seed):
     assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
 
 
+def test_binning_stable_over_a_reused_workspace():
+    """One workspace reused over inputs in cell order, shuffled, and
+    reversed by cells or by particles (each cell's particles then one run
+    of one wavefront, or spread over many), as the step reuses its own:
+    every call the stable argsort."""
+    from peel_mpm_amd import kernels
+    ng, ncell, ppc = 64, 5000, 8           # 8 particles per cell: no cell straddles a wavefront in cell order
+    g = np.random.default_rng(21)
+    flat = g.choice((ng - 4) ** 3, size=ncell, replace=False)
+    c = np.stack([flat // (ng - 4) ** 2, (flat // (ng - 4)) % (ng - 4), flat % (ng - 4)], 1) + 2
+    x = ((np.repeat(c, ppc, 0) + g.uniform(0.05, 0.95, (ncell * ppc, 3))) / ng).astype(np.float32)
+    x = x[np.argsort(expected_cell_keys(torch.from_numpy(x), ng), kind="stable")]
+    grp = np.arange(ncell * ppc).reshape(ncell, ppc)
+    inputs = {"cell order": x, "shuffled": x[g.permutation(len(x))], "cell order again": x,
+              "cells reversed": x[grp[::-1].reshape(-1)], "particles reversed": x[::-1].copy()}
+    ws = kernels.bin_workspace(len(x), ng)
+    for name, xi in inputs.items():
+        print(f"bin reuse {name}", flush=True)
+        key, perm, _ = kernels.bin_particles(torch.from_numpy(np.ascontiguousarray(xi)).to(DEV), ng, stable=True, ws=ws)
+        want = expected_cell_keys(torch.from_numpy(xi), ng)
+        assert np.array_equal(key.cpu().numpy(), want), name
+        assert np.array_equal(perm.cpu().numpy(), np.argsort(want, kind="stable")), name
+
+
 def test_binning_unstable_is_a_permutation_grouped_by_bucket():
     from peel_mpm_amd import kernels
     g = np.random.default_rng(9)
