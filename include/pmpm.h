@@ -104,7 +104,22 @@ typedef struct {
      * ws as that forward call left them (same inputs, same ws, nothing run on
      * ws in between) instead of recomputing them. */
     int32_t keep_for_adjoint;
+    /* Optional DEVICE pointer to a pmpm_boundary_table.  When set, the
+     * kernels read the colliders and velocity-BC values from it when they
+     * run, instead of from colliders[] and vel_bc[] above (n_colliders and
+     * n_vel_bc still count them).  A step captured in a HIP graph then
+     * follows moving colliders and time-dependent velocities: the caller
+     * writes the table before each replay.  No counterpart in the reference
+     * (its closures run on the host every step, anisompm.py:601, :744). */
+    const struct pmpm_boundary_table *boundary_table;
 } pmpm_step_params;
+
+/* The device-resident form of the step's boundary values (see
+ * pmpm_step_params.boundary_table). */
+typedef struct pmpm_boundary_table {
+    pmpm_collider colliders[PMPM_MAX_COLLIDERS];
+    double vel_bc[PMPM_MAX_VEL_BC][3];
+} pmpm_boundary_table;
 
 enum { PMPM_DAMAGE_EXPLICIT = 0, PMPM_DAMAGE_PCG = 1 };
 
@@ -131,7 +146,13 @@ typedef struct {
 /* ---- workspace ---------------------------------------------------------- */
 /* Bytes of workspace for up to max_n particles over n_scenes grids of n_grid^3. */
 size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype);
-/* Zero the parts of the workspace whose zero state the step maintains. */
+/* Zero the workspace and record, by its address, the shape it was set up
+ * for.  Every call that takes ws afterwards (steps, adjoint, pmpm_bin, the
+ * stats readers) needs the same n_grid, n_scenes and dtype and at most max_n
+ * particles; any n <= max_n is served (the sections that depend on the
+ * particle count are laid out last), anything else returns PMPM_EARG before
+ * work is enqueued, as does a ws this call never set up.  Call it again for
+ * a buffer reused with another shape. */
 int pmpm_workspace_init(void *ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t n_scenes,
                         int32_t dtype, void *stream);
 
@@ -251,8 +272,8 @@ int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
 const char *pmpm_last_error(void);
 const char *pmpm_version(void);
 /* sizeof of pmpm_collider, pmpm_step_params, pmpm_particles_in,
- * pmpm_particles_out, pmpm_frame_in, pmpm_frame_obs, pmpm_pcg64 (in that
- * order) for binding-layout checks; returns the number of sizes written (at
+ * pmpm_particles_out, pmpm_frame_in, pmpm_frame_obs, pmpm_pcg64,
+ * pmpm_boundary_table (in that order) for binding-layout checks; returns the number of sizes written (at
  * most n). */
 int pmpm_abi_sizes(size_t *sizes, int32_t n);
 

@@ -49,7 +49,13 @@ class StepParams(C.Structure):
                 ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32),
                 ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p),
                 ("damage_solver", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32),
-                ("keep_for_adjoint", C.c_int32)]
+                ("keep_for_adjoint", C.c_int32), ("boundary_table", C.c_void_p)]
+
+
+class BoundaryTable(C.Structure):
+    """pmpm_boundary_table: colliders and BC velocities read by the kernels
+    when they run (StepParams.boundary_table holds its device address)."""
+    _fields_ = [("colliders", Collider * MAX_COLLIDERS), ("vel_bc", (C.c_double * 3) * MAX_VEL_BC)]
 
 
 _IN_PTRS = ("x", "v", "F", "C", "d", "lap", "D_hist", "vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0",
@@ -127,10 +133,10 @@ def lib():
     L.pmpm_fill_bounds.argtypes = [i32, i32, vp, i64, C.POINTER(dbl), dbl, dbl, C.POINTER(dbl), C.POINTER(i64), vp, sz, vp]
     L.pmpm_fill_interior.argtypes = [i32, i32, vp, i64, C.POINTER(dbl), dbl, dbl, C.POINTER(dbl), C.POINTER(i64), dbl,
                                      C.POINTER(Pcg64), vp, i64, C.POINTER(i64), vp, sz, vp]
-    sizes = (sz * 7)()
-    L.pmpm_abi_sizes(sizes, 7)
+    sizes = (sz * 8)()
+    L.pmpm_abi_sizes(sizes, 8)
     mine = (C.sizeof(Collider), C.sizeof(StepParams), C.sizeof(ParticlesIn), C.sizeof(ParticlesOut), C.sizeof(FrameIn),
-            C.sizeof(FrameObs), C.sizeof(Pcg64))
+            C.sizeof(FrameObs), C.sizeof(Pcg64), C.sizeof(BoundaryTable))
     if tuple(sizes) != mine:
         raise ImportError(f"libpmpm ABI mismatch: library struct sizes {tuple(sizes)} != binding {mine}")
     _lib = L

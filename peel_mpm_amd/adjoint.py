@@ -60,9 +60,15 @@ def _lib_adj():
 
 class StepRunner:
     """Functional native step over explicit state dicts, using the materials,
-    configuration and VelocityBC list of an AnisoMPM ``template``."""
+    configuration and VelocityBC list of an AnisoMPM ``template``.  Substep k
+    runs at time time0 + k dt, with time0 (default) the template's own
+    time, template._step * dt, as its next substep would.  With
+    exact_sigma_eig the forward steps take the Cauchy eigenpairs from a
+    second Jacobi as the template's do; the adjoint differentiates the closed
+    form from the polar stretch, which is the same function (the two agree
+    to rounding)."""
 
-    def __init__(self, template: AnisoMPM):
+    def __init__(self, template: AnisoMPM, time0: float | None = None):
         sim = template
         if sim.f_clamp is not None or sim.colliders or sim.damage_every != 1:
             raise NotImplementedError("the adjoint supports damage_every=1, no f_clamp and no colliders")
@@ -73,6 +79,7 @@ class StepRunner:
         if not all(isinstance(b, VelocityBC) for b in sim.particle_bc):
             raise NotImplementedError("the adjoint takes VelocityBC particle boundary conditions only")
         self.sim = sim
+        self.time0 = float(sim._step * sim.dt if time0 is None else time0)
         self.dev, self.dtype, self.n = sim.device, sim.dtype, sim.n
         self.do_damage = bool(sim.allow.any())
         self.allow = sim.allow.contiguous()
@@ -102,7 +109,7 @@ class StepRunner:
 
     def _step_params(self, k):
         sim = self.sim
-        bc_id, bc_vals = sim._velocity_bcs(k * sim.dt)
+        bc_id, bc_vals = sim._velocity_bcs(self.time0 + k * sim.dt)
         p = _lib.StepParams()
         p.dtype, p.n_grid = sim._code, sim.n_grid
         p.grid_lim, p.dx, p.inv_dx, p.dt = sim.grid_lim, sim.dx, sim.inv_dx, sim.dt
@@ -114,6 +121,7 @@ class StepRunner:
         for j, v in enumerate(bc_vals):
             p.vel_bc[j][:] = v
         p.stable_sort = 1
+        p.exact_sigma_eig = int(sim.exact_sigma_eig)
         return p, bc_id
 
     def _pin(self, s, bc_id):
@@ -224,11 +232,13 @@ class _Rollout(torch.autograd.Function):
         return adj["v"], grads["mu"], grads["lam"], grads["A0"], grads["sigma_c"], None, None, None, None
 
 
-def rollout(sim: AnisoMPM, v0, mu, lam, A0, steps: int, every: int = 10, sigma_c=None):
+def rollout(sim: AnisoMPM, v0, mu, lam, A0, steps: int, every: int = 10, sigma_c=None, time0=None):
     """Final positions after ``steps`` substeps from ``sim``'s current state,
     differentiable in v0, mu, lam, A0 and sigma_c (the damage threshold;
-    default: sim.sigma_c), with a checkpoint every ``every`` steps."""
-    runner = StepRunner(sim)
+    default: sim.sigma_c), with a checkpoint every ``every`` steps.  The
+    substeps run at time0 + k dt (default time0: sim._step * dt), as
+    sim.run_frame(steps, time0) would."""
+    runner = StepRunner(sim, time0)
     state0 = {k: getattr(sim, k) for k in STATE}
     if sigma_c is None:
         sigma_c = sim.sigma_c
@@ -245,12 +255,13 @@ def material_tensors(E, nu, a0, alpha):
     return mu, lam, A0
 
 
-def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10, reuse: bool = True):
+def state_vjp(sim: AnisoMPM, steps: int, x_bar, every: int = 10, reuse: bool = True, time0=None):
     """Adjoint of the whole initial state and of mu, lam, A0, sigma_c for the
     vector x_bar on the positions after ``steps`` substeps (no autograd
-    graph).  reuse=False makes every adjoint step recompute its forward
-    binning and grid instead of reusing the segment's workspaces."""
-    runner = StepRunner(sim)
+    graph), the substeps at time0 + k dt (default time0: sim._step * dt).
+    reuse=False makes every adjoint step recompute its forward binning and
+    grid instead of reusing the segment's workspaces."""
+    runner = StepRunner(sim, time0)
     s = {k: getattr(sim, k).contiguous() for k in STATE}
     ckpt = {0: s}
     for k in range(steps):

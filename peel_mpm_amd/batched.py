@@ -7,12 +7,20 @@ and arena slots are scene-major, and a per-scene particle offset table
 (``scene_offsets``) tells the binning kernel which grid a particle lives on.
 Every kernel of the step is one persistent launch over all scenes' work.
 
-With ``use_graph=True`` the step is captured once as a HIP graph (through
-torch.cuda.CUDAGraph) on two ping-pong state buffers and replayed; this
-needs time-invariant boundary conditions (constant VelocityBC velocities and
-static colliders) and damage_every == 1, and it keeps phi/phi_shear/
-tau_last/J_last in persistent buffers updated in place.  State attributes
-point at the current ping-pong buffer: do not hold them across steps.
+Per step, before the kernels run, the host writes two small device tables
+that the kernels read when they run: the colliders and BC velocities at the
+step's time (``DeviceBoundaryTable``) and the per-scene damage gate (the
+scene's ``allow.any()`` on the steps ``damage_every`` selects, recomputed on
+the device whenever ``allow`` changes; anisompm.py:647).  With
+``use_graph=True`` the step is captured once as a HIP graph (through
+torch.cuda.CUDAGraph) on two ping-pong state buffers and replayed, and
+because those tables are written outside the graph, moving colliders,
+time-dependent velocities, in-place edits of ``allow`` and damage_every > 1
+all hold under replay.  The graph is captured again when a parameter tensor
+is replaced (rather than edited in place) or the collider count changes.
+phi/phi_shear/tau_last/J_last are persistent buffers updated in place.
+State attributes point at the current ping-pong buffer: do not hold them
+across steps.
 """
 from __future__ import annotations
 
@@ -21,7 +29,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from .boundaries import HalfspaceCollider, SphereCollider, VelocityBC
+from .boundaries import DeviceBoundaryTable, HalfspaceCollider, SphereCollider, VelocityBC
 
 _STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
 _FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "residual", "A0", "n_interface")
@@ -111,11 +119,14 @@ class BatchedAnisoMPM:
         if len(vel_slots) > _lib.MAX_VEL_BC:
             raise ValueError(f"at most {_lib.MAX_VEL_BC} distinct BC velocities in a batch")
         self._bc_id = ids
-        # per-scene damage gate (anisompm.py:647): allow.any() of the scene, or
-        # all-off on the steps that damage_every skips
-        self._scene_allow = torch.tensor([bool(self.allow[off[s]:off[s + 1]].any()) for s in range(self.n_scenes)],
-                                         dtype=torch.uint8, device=dev)
+        # per-scene damage gate (anisompm.py:647), read by the kernels from
+        # _gate; _scene_allow follows allow (see _refresh_gate)
+        self._allow_key = None
+        self._scene_allow = None
         self._scene_none = torch.zeros(self.n_scenes, dtype=torch.uint8, device=dev)
+        self._gate = torch.zeros(self.n_scenes, dtype=torch.uint8, device=dev)
+        self._btab = DeviceBoundaryTable(dev)
+        self._graph_key = None
         L = _lib.lib()
         nbytes = L.pmpm_workspace_bytes(n, self.n_grid, self.n_scenes, self._code)
         self._ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
@@ -130,7 +141,22 @@ class BatchedAnisoMPM:
             setattr(self, k, v)
 
     # ------------------------------------------------------------------ step
-    def _params(self, time):
+    def _refresh_gate(self):
+        """Write this step's per-scene damage gate into _gate (device ops on
+        the current stream, no host synchronisation): allow.any() per scene,
+        recomputed only when allow was replaced or edited in place, and all
+        zero on the steps damage_every skips."""
+        a = self.allow
+        key = (id(a), a.data_ptr(), a._version)
+        if key != self._allow_key:
+            c = torch.zeros(self.n + 1, dtype=torch.int64, device=self.device)
+            c[1:] = torch.cumsum(a, 0)
+            self._scene_allow = (c[self._off_dev[1:]] - c[self._off_dev[:-1]] > 0).to(torch.uint8)
+            self._allow_key = key
+        on = self._step % self.damage_every == 0
+        self._gate.copy_(self._scene_allow if on else self._scene_none)
+
+    def _params(self):
         p = _lib.StepParams()
         p.dtype, p.n_grid = self._code, self.n_grid
         p.grid_lim, p.dx, p.inv_dx, p.dt = self.grid_lim, self.dx, self.inv_dx, self.dt
@@ -140,26 +166,18 @@ class BatchedAnisoMPM:
             p.use_fclamp, p.fclamp_lo, p.fclamp_hi = 1, float(self.f_clamp[0]), float(self.f_clamp[1])
         p.do_damage = 1
         p.dt_damage = self.dt * self.damage_every
-        if not all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders):
-            raise ValueError("batched scenes take device colliders (sphere_collider / halfspace_collider) only")
-        if len(self.colliders) > _lib.MAX_COLLIDERS:
-            raise ValueError(f"at most {_lib.MAX_COLLIDERS} colliders")
         p.n_colliders = len(self.colliders)
-        for k, c in enumerate(self.colliders):
-            c.describe(time, p.colliders[k])
         p.n_vel_bc = len(self._bc_fns)
-        for k, b in enumerate(self._bc_fns):
-            p.vel_bc[k][:] = b.value(time)
+        p.boundary_table = self._btab.ptr
         p.exact_sigma_eig = int(self.exact_sigma_eig)
         p.stable_sort = int(self.stable_sort)
         p.n_scenes = self.n_scenes
         p.scene_offsets = self._off_dev.data_ptr()
-        on = self._step % self.damage_every == 0
-        p.scene_damage = (self._scene_allow if on else self._scene_none).data_ptr()
+        p.scene_damage = self._gate.data_ptr()
         return p
 
-    def _launch(self, src, dst, time):
-        p = self._params(time)
+    def _launch(self, src, dst):
+        p = self._params()
         pin = _lib.ParticlesIn()
         pin.n = self.n
         for k in _STATE:
@@ -177,32 +195,46 @@ class BatchedAnisoMPM:
         _lib.check(_lib.lib().pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), self._ws.data_ptr(),
                                            self._ws.numel(), _lib.stream_ptr(self.device)), "pmpm_substep")
 
-    def _graph_ok(self):
-        # collider positions are frozen at capture time; BC velocities must be constants
-        return self.damage_every == 1 and all(not callable(b.velocity) for b in self._bc_fns)
+    def _check_colliders(self):
+        if not all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders):
+            raise ValueError("batched scenes take device colliders (sphere_collider / halfspace_collider) only")
+        if len(self.colliders) > _lib.MAX_COLLIDERS:
+            raise ValueError(f"at most {_lib.MAX_COLLIDERS} colliders")
+
+    def _capture_key(self):
+        """What a captured graph holds by value or by address besides the
+        ping-pong buffers: the parameter tensors and the collider count."""
+        return tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS + ("allow", "interface_mask")) + \
+            (len(self.colliders),)
 
     def substep(self, time: float = 0.0):
         if not self._built:
             self._build()
+        self._check_colliders()
         src, dst = self._bufs[self._cur], self._bufs[1 - self._cur]
+        # the tables the kernels read when they run, written before the step
+        self._btab.write(self.colliders, self._bc_fns, time)
+        self._refresh_gate()
         if self.use_graph:
-            if not self._graph_ok():
-                raise ValueError("graph mode needs constant VelocityBC velocities and damage_every == 1")
+            key = self._capture_key()
+            if key != self._graph_key:
+                self._graphs = [None, None]
+                self._graph_key = key
             g = self._graphs[self._cur]
             if g is None:
                 # warm up once eagerly on a side stream, then capture src -> dst
                 s = torch.cuda.Stream(device=self.device)
                 s.wait_stream(torch.cuda.current_stream(self.device))
                 with torch.cuda.stream(s):
-                    self._launch(src, dst, time)
+                    self._launch(src, dst)
                 torch.cuda.current_stream(self.device).wait_stream(s)
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g):
-                    self._launch(src, dst, time)
+                    self._launch(src, dst)
                 self._graphs[self._cur] = g
             g.replay()
         else:
-            self._launch(src, dst, time)
+            self._launch(src, dst)
         self._cur = 1 - self._cur
         self._bind()
         self._step += 1

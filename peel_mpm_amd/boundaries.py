@@ -15,6 +15,8 @@ no masked host write (and no host sync) happens per step.
 """
 from __future__ import annotations
 
+import ctypes as C
+
 import torch
 
 from . import _lib
@@ -114,3 +116,51 @@ class VelocityBC:
     def __call__(self, sim, time):
         sim.v[torch.as_tensor(self.mask, device=sim.v.device)] = torch.tensor(self.value(time), dtype=sim.v.dtype,
                                                                               device=sim.v.device)
+
+
+def describe_boundaries(colliders, vel_bcs, time: float, table: _lib.BoundaryTable):
+    """Fill a pmpm_boundary_table with the device colliders and VelocityBC
+    values at ``time``."""
+    if len(colliders) > _lib.MAX_COLLIDERS:
+        raise ValueError(f"at most {_lib.MAX_COLLIDERS} colliders")
+    if len(vel_bcs) > _lib.MAX_VEL_BC:
+        raise ValueError(f"at most {_lib.MAX_VEL_BC} VelocityBC values")
+    for k, c in enumerate(colliders):
+        c.describe(time, table.colliders[k])
+    for k, b in enumerate(vel_bcs):
+        table.vel_bc[k][:] = b.value(time)
+
+
+class DeviceBoundaryTable:
+    """The step's colliders and BC velocities in device memory
+    (``StepParams.boundary_table``), rewritten before every step, so a step
+    replayed from a HIP graph follows moving colliders and time-dependent
+    velocities.  The host side is a ring of pinned buffers: a slot is reused
+    only after the copy out of it has run (its event), so the host never
+    overwrites values a pending copy still has to read."""
+
+    def __init__(self, device, slots: int = 4):
+        nb = C.sizeof(_lib.BoundaryTable)
+        self.device = torch.device(device)
+        self.dev = torch.zeros(nb, dtype=torch.uint8, device=self.device)
+        self._host = [torch.zeros(nb, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        self._events = [None] * slots
+        self._i = 0
+        self.table = _lib.BoundaryTable()
+
+    @property
+    def ptr(self) -> int:
+        return self.dev.data_ptr()
+
+    def write(self, colliders, vel_bcs, time: float):
+        """Enqueue (on the current stream) the copy of the values at ``time``."""
+        describe_boundaries(colliders, vel_bcs, time, self.table)
+        i = self._i
+        self._i = (i + 1) % len(self._host)
+        if self._events[i] is not None:
+            self._events[i].synchronize()
+        C.memmove(self._host[i].data_ptr(), C.addressof(self.table), C.sizeof(self.table))
+        self.dev.copy_(self._host[i], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        self._events[i] = ev

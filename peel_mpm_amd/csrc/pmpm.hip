@@ -35,7 +35,9 @@
 
 #include <algorithm>
 #include <chrono>
+#include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/pmpm.h"
@@ -92,12 +94,13 @@ struct Timing {
     }
 };
 Timing g_timing;
+thread_local bool t_untimed = false;    // set while a no-allocation step enqueues (no event creation)
 struct TimedScope {
     int group;
     hipStream_t st;
     hipEvent_t stop = nullptr;
     TimedScope(int g, hipStream_t s) : group(g), st(s) {
-        if (!g_timing.on) return;
+        if (!g_timing.on || t_untimed) return;
         hipEvent_t start = g_timing.get();
         stop = g_timing.get();
         if (!start || !stop) { stop = nullptr; return; }
@@ -116,12 +119,18 @@ inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
 pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256};
 
-// Workspace layout (byte offsets).  The step keeps count, bflag and mark at
-// zero between steps: whoever reads them clears them.  count/start are per
-// cell (bucket-major; start is kept for the cells of non-empty buckets),
-// btot/bstart/bflag/alist/slotof/mark/blist per bucket or grid block.  arena holds one
-// 6 x 343 slot per non-empty bucket (m, mv, sum w d, sum w), in active-list
-// order.
+// Workspace layout (byte offsets).  The step keeps count, bflag, mark and
+// misc at zero between steps: whoever reads them clears them.  count/start
+// are per cell (bucket-major; start is kept for the cells of non-empty
+// buckets), btot/bstart/bflag/alist/slotof/mark/blist per bucket or grid
+// block.  arena holds one 6 x 343 slot per non-empty bucket (m, mv, sum w d,
+// sum w), in active-list order.
+//
+// Every section whose size does not depend on the particle count comes
+// first, so a workspace laid out for max_n particles serves any n <= max_n
+// at the same offsets for them -- in particular for the zero-state sections.
+// The sections sized by n (arena, then the per-particle ones) follow; the
+// step rewrites them in full every call.
 struct Layout {
     size_t arena, gv, active, count, start, btot, bstart, bflag, tiles, mark, alist, slotof, blist, key, slot, perm, inv, rec,
         dsrt, misc, stats, pcgd, pcgr, gmw, total;
@@ -130,42 +139,79 @@ struct Layout {
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
         size_t ts = dtype == PMPM_F64 ? 8 : 4;
         if (ns < 1) ns = 1;
+        const size_t np = (size_t)(n > 0 ? n : 1);
         Gs = (long long)ng * ng * ng;
         G = Gs * ns;
         NBs = (long long)nblk(ng) * nblk(ng) * nblk(ng);
         NB = NBs * ns;
         NC = NB * 64;                            // cells, bucket-major
         ntiles = (NB + BTILE - 1) / BTILE;       // bucket-scan tiles
-        max_slots = NB < (long long)(n > 0 ? n : 1) ? NB : (long long)(n > 0 ? n : 1);
+        max_slots = NB < (long long)np ? NB : (long long)np;
         size_t o = 0;
-        arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
-        gv = o;     o = al(o + (size_t)G * 4 * ts);
-        active = o; o = al(o + (size_t)G);
+        // ---- sections independent of n
+        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2] grid partials,
+                                             // [3] some stencil leaves the grid (this step), [4] its raw flag
         count = o;  o = al(o + (size_t)NC * 4);
+        bflag = o;  o = al(o + (size_t)NB * 4);
+        mark = o;   o = al(o + (size_t)NB * 4);
         start = o;  o = al(o + (size_t)(NC + 1) * 4);
         btot = o;   o = al(o + (size_t)NB * 4);
         bstart = o; o = al(o + (size_t)NB * 4);
-        bflag = o;  o = al(o + (size_t)NB * 4);
         tiles = o;  o = al(o + (size_t)ntiles * 2 * 4);
-        mark = o;   o = al(o + (size_t)NB * 4);
         alist = o;  o = al(o + (size_t)NB * 4);
         slotof = o; o = al(o + (size_t)NB * 4);
         blist = o;  o = al(o + (size_t)NB * 4);
-        key = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);
-        slot = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
-        perm = o;   o = al(o + (size_t)(n > 0 ? n : 1) * 4);
-        inv = o;    o = al(o + (size_t)(n > 0 ? n : 1) * 4);              // sorted position of each particle
-        rec = o;    o = al(o + (size_t)(n > 0 ? n : 1) * P2G_REC * ts);   // P2G records, sorted order
-        dsrt = o;   o = al(o + (size_t)(n > 0 ? n : 1) * ts);             // post-update d, sorted order
-        misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2] grid partials,
-                                             // [3] some stencil leaves the grid (this step), [4] its raw flag
         stats = o;  o = al(o + (size_t)GRID_WG_COUNT * 16);   // per grid work-group: active q, c nodes
+        pcgr = o;   o = al(o + (size_t)(GRID_WG_COUNT * 3 + PCG_NSCAL) * 8);   // PCG partial dots + scalars
+        gv = o;     o = al(o + (size_t)G * 4 * ts);
+        active = o; o = al(o + (size_t)G);
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
         gmw = o;    o = al(o + (size_t)G * 2 * ts);                // per node (m, sum w) kept for the adjoint
-        pcgr = o;   o = al(o + (size_t)(GRID_WG_COUNT * 3 + PCG_NSCAL) * 8);   // PCG partial dots + scalars
+        // ---- sections sized by n
+        arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
+        key = o;    o = al(o + np * 4);
+        slot = o;   o = al(o + np * 4);
+        perm = o;   o = al(o + np * 4);
+        inv = o;    o = al(o + np * 4);                   // sorted position of each particle
+        rec = o;    o = al(o + np * P2G_REC * ts);        // P2G records, sorted order
+        dsrt = o;   o = al(o + np * ts);                  // post-update d, sorted order
         total = o;
     }
+    // the zero-state sections: one contiguous range at the front
+    size_t zero_end() const { return start; }
 };
+
+// Workspaces set up by pmpm_workspace_init, by address: the shape each was
+// laid out for.  A call checks its workspace here before it enqueues
+// anything, so a buffer laid out for another grid, scene count or dtype, or
+// for fewer particles, is refused instead of read at shifted offsets.
+struct WsShape {
+    int64_t max_n;
+    int32_t ng, ns, dtype;
+    size_t bytes;
+};
+std::mutex g_ws_mu;
+std::unordered_map<uintptr_t, WsShape> g_ws;
+
+void ws_register(const void* ws, const WsShape& s) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws[(uintptr_t)ws] = s;
+}
+
+int ws_check(const void* ws, int64_t n, int ng, int ns, int dtype) {
+    if (ns < 1) ns = 1;
+    WsShape s;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto it = g_ws.find((uintptr_t)ws);
+        if (it == g_ws.end()) return fail(PMPM_EARG, "%sworkspace was not set up by pmpm_workspace_init", "");
+        s = it->second;
+    }
+    if (s.ng != ng || s.ns != ns || s.dtype != dtype)
+        return fail(PMPM_EARG, "%sworkspace was set up for another grid, scene count or dtype (n_grid %lld)", "", s.ng);
+    if (n > s.max_n) return fail(PMPM_EARG, "%sworkspace was set up for at most %lld particles", "", (long long)s.max_n);
+    return PMPM_OK;
+}
 
 // ------------------------------------------------------------------ binning
 // Sort key of a particle: its cell, numbered bucket-major -- the 4x4x4-cell
@@ -482,7 +528,26 @@ template <typename T> struct StepArgs {
     int pcg_max;
     pmpm_collider col[PMPM_MAX_COLLIDERS];
     T bcv[PMPM_MAX_VEL_BC][3];
+    // optional device copy of the colliders and BC velocities, read at kernel
+    // time instead of the by-value copies above (a replayed graph then sees
+    // values the host wrote before the replay)
+    const pmpm_boundary_table* btab;
 };
+
+// Prescribed velocity of a particle with velocity-BC slot bc (1-based; 0: none).
+template <typename T> __device__ __forceinline__ void bc_velocity(const StepArgs<T>& A, int bc, T vv[3]) {
+    if (bc <= 0) return;
+    if (A.btab) {
+        const double* t = A.btab->vel_bc[bc - 1];
+        vv[0] = T(t[0]); vv[1] = T(t[1]); vv[2] = T(t[2]);
+        return;
+    }
+    // compile-time indices only, so the by-value kernel arguments are never
+    // copied to scratch for a dynamic index
+#pragma unroll
+    for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
+        if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
+}
 
 // Persistent-bucket schedule: the non-empty bucket list is cut into 8
 // contiguous parts, one per XCD (work-groups are dealt to the 8 XCDs round
@@ -763,11 +828,7 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     sk = sk * T(4);
     sk = sk * A.inv_dx;
     sk = sk * A.inv_dx;
-    // velocity BC slot: compile-time indices only, so the by-value kernel
-    // arguments are never copied to scratch for a dynamic index
-#pragma unroll
-    for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
-        if (bc == k + 1) { vv[0] = A.bcv[k][0]; vv[1] = A.bcv[k][1]; vv[2] = A.bcv[k][2]; }
+    bc_velocity(A, bc, vv);
     T* r = A.rec + (size_t)P2G_REC * i;
     r[0] = xp[0]; r[1] = xp[1]; r[2] = xp[2];
 #pragma unroll
@@ -1103,7 +1164,7 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     __shared__ pmpm_collider s_col[PMPM_MAX_COLLIDERS];
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) s_col[k] = A.col[k];
+        for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) s_col[k] = A.btab ? A.btab->colliders[k] : A.col[k];
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -2151,14 +2212,18 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.export_active = p->export_active;
     A.n_col = p->n_colliders;
     for (int k = 0; k < p->n_colliders; ++k) A.col[k] = p->colliders[k];
+    A.btab = p->boundary_table;
     A.n_bc = p->n_vel_bc;
     for (int k = 0; k < p->n_vel_bc; ++k)
         for (int a = 0; a < 3; ++a) A.bcv[k][a] = T(p->vel_bc[k][a]);
     return A;
 }
 
-int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, size_t ws_bytes,
-               int64_t& max_n_out) {
+// Arguments of a step call.  ws: the caller's workspace, checked against
+// the shape pmpm_workspace_init set it up for (null: the library's own
+// scratch, laid out per call).
+int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, const void* ws,
+               size_t ws_bytes, int64_t& max_n_out) {
     if (!p || !in || !out) return fail(PMPM_EARG, "%snull argument", "");
     if (p->dtype != PMPM_F32 && p->dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", p->dtype);
     if (p->n_grid < 4 || p->n_grid > 1024) return fail(PMPM_EARG, "%sn_grid out of range: %lld", "", p->n_grid);
@@ -2176,6 +2241,10 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     }
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
+    if (ws) {
+        const int rc = ws_check(ws, in->n, p->n_grid, p->n_scenes, p->dtype);
+        if (rc) return rc;
+    }
     if (in->n > 0) {
         const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, in->vol0, in->mass, in->mu, in->lam,
                               in->residual, in->interface_mask, out->x, out->v, out->F, out->C, out->lap, out->tau_last,
@@ -2198,7 +2267,7 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
 
 template <typename T>
 int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
-               int exact_sigma) {
+               int exact_sigma, bool host_poll = true) {
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
                         p->n_scenes, L, ws, st);
@@ -2231,7 +2300,9 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         // the ones after convergence exit at their first load.
         hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
         HIPCHK(hipStreamIsCapturing(st, &cap));
-        const bool capturing = cap != hipStreamCaptureStatusNone;
+        // enqueue every iteration under capture, and when the caller allows no
+        // host-side set-up (the pinned flag is allocated on first use)
+        const bool capturing = cap != hipStreamCaptureStatusNone || !host_poll;
         PcgFlag* pf = capturing ? nullptr : pcg_flag();
         unsigned long long* hflag = nullptr;
         unsigned seq = 0;
@@ -2344,10 +2415,11 @@ const char* pmpm_last_error(void) { return g_err; }
 const char* pmpm_version(void) { return "pmpm 0.1 gfx950"; }
 
 int pmpm_abi_sizes(size_t* sizes, int32_t n) {
-    const size_t all[7] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
-                           sizeof(pmpm_particles_out), sizeof(pmpm_frame_in), sizeof(pmpm_frame_obs), sizeof(pmpm_pcg64)};
+    const size_t all[8] = {sizeof(pmpm_collider), sizeof(pmpm_step_params), sizeof(pmpm_particles_in),
+                           sizeof(pmpm_particles_out), sizeof(pmpm_frame_in), sizeof(pmpm_frame_obs), sizeof(pmpm_pcg64),
+                           sizeof(pmpm_boundary_table)};
     int k = 0;
-    for (; k < n && k < 7; ++k) sizes[k] = all[k];
+    for (; k < n && k < 8; ++k) sizes[k] = all[k];
     return k;
 }
 
@@ -2359,9 +2431,13 @@ size_t pmpm_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t n_scenes, int
 int pmpm_workspace_init(void* ws, size_t ws_bytes, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
                         void* stream) {
     if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
+    if (n_grid < 4 || n_grid > 1024 || max_n < 0 || max_n > 0x7fffffffLL || n_scenes < 0)
+        return fail(PMPM_EARG, "%sworkspace shape out of range (max_n %lld)", "", (long long)max_n);
+    if (dtype != PMPM_F32 && dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", dtype);
     Layout L(max_n, n_grid, dtype, n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     HIPCHK(hipMemsetAsync(ws, 0, L.total, (hipStream_t)stream));
+    ws_register(ws, WsShape{max_n, n_grid, n_scenes < 1 ? 1 : n_scenes, dtype, ws_bytes});
     return PMPM_OK;
 }
 
@@ -2375,6 +2451,7 @@ int pmpm_grid_views(int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dty
 
 int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype, int64_t* n_active_q,
                          int64_t* n_active_c, void* stream) {
+    if (int rc = ws_check(ws, 0, n_grid, n_scenes, dtype)) return rc;
     Layout L(max_n, n_grid, dtype, n_scenes);
     std::vector<unsigned long long> h(2 * GRID_WG_COUNT);
     int misc[4] = {0, 0, 0, 0};
@@ -2423,6 +2500,7 @@ int pmpm_get_launch_config(pmpm_launch_config* cfg) {
 int pmpm_damage_solve_info(const void* ws, int64_t max_n, int32_t n_grid, int32_t n_scenes, int32_t dtype,
                            int32_t* iterations, double* rel_residual, void* stream) {
     if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
+    if (int rc = ws_check(ws, 0, n_grid, n_scenes, dtype)) return rc;
     Layout L(max_n, n_grid, dtype, n_scenes);
     double h[PCG_NSCAL];
     HIPCHK(hipMemcpyAsync(h, (const char*)ws + L.pcgr + (size_t)3 * GRID_WG_COUNT * 8, sizeof h, hipMemcpyDeviceToHost,
@@ -2436,7 +2514,7 @@ int pmpm_damage_solve_info(const void* ws, int64_t max_n, int32_t n_grid, int32_
 int pmpm_substep_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
                        size_t ws_bytes, void* stream) {
     int64_t n;
-    int rc = check_step(p, in, out, ws_bytes, n);
+    int rc = check_step(p, in, out, ws, ws_bytes, n);
     if (rc) return rc;
     if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
     return p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, (hipStream_t)stream, p->exact_sigma_eig)
@@ -2446,8 +2524,9 @@ int pmpm_substep_begin(const pmpm_step_params* p, const pmpm_particles_in* in, c
 int pmpm_substep_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, void* ws,
                      size_t ws_bytes, void* stream) {
     int64_t n;
-    int rc = check_step(p, in, out, ws_bytes, n);
+    int rc = check_step(p, in, out, ws, ws_bytes, n);
     if (rc) return rc;
+    if (!ws) return fail(PMPM_EARG, "%snull workspace", "");
     return p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, (hipStream_t)stream)
                                 : step_end<float>(p, in, out, (char*)ws, (hipStream_t)stream);
 }
@@ -2491,9 +2570,9 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
                          const pmpm_adjoint_in* gin, const pmpm_adjoint_out* gout, void* ws, size_t ws_bytes, void* aws,
                          size_t aws_bytes, void* stream) {
     int64_t n;
-    int rc = check_step(p, in, fwd_out, ws_bytes, n);
-    if (rc) return rc;
     if (!ws || !aws || !gin || !gout) return fail(PMPM_EARG, "%snull argument", "");
+    int rc = check_step(p, in, fwd_out, ws, ws_bytes, n);
+    if (rc) return rc;
     if (aws_bytes < pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype))
         return fail(PMPM_ESIZE, "%sadjoint workspace too small, need %lld bytes", "",
                     (long long)pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype));
@@ -2519,6 +2598,7 @@ int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv
     if (n_grid < 4 || n_grid > 1024 || n < 0 || n > 0x7fffffffLL) return fail(PMPM_EARG, "%sbad size %lld", "", n);
     Layout L(n, n_grid, dtype);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
+    if (int rc = ws_check(ws, n, n_grid, 1, dtype)) return rc;
     hipStream_t st = (hipStream_t)stream;
     int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st)
                                : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st);

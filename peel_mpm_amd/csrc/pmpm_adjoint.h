@@ -497,9 +497,7 @@ __global__ void __launch_bounds__(WG) k_adj_gather(StepArgs<T> A, AdjArgs<T> B) 
     const T mass = A.mass[p];
     T vin[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
     const int bc = A.bc_id ? A.bc_id[p] : 0;
-#pragma unroll
-    for (int k = 0; k < PMPM_MAX_VEL_BC; ++k)
-        if (bc == k + 1) { vin[0] = A.bcv[k][0]; vin[1] = A.bcv[k][1]; vin[2] = A.bcv[k][2]; }
+    bc_velocity(A, bc, vin);
     // the affine matrix (sk tau + m C) and the post-update d of the forward
     // step, from the P2G record the recompute just wrote at the particle's
     // sorted position

@@ -56,6 +56,29 @@ def test_workspace_size_and_argument_errors():
     assert rc == -1 and b"collider" in L.pmpm_last_error()
 
 
+def test_workspace_not_set_up_is_refused_before_any_launch():
+    """A workspace pmpm_workspace_init never set up is refused by every call
+    that takes one, on the host, before anything is enqueued (the address
+    below is never dereferenced)."""
+    from peel_mpm_amd import _lib
+    L = _lib.lib()
+    fake = 0x7F0000000000
+    p, pin, pout = _lib.StepParams(), _lib.ParticlesIn(), _lib.ParticlesOut()
+    p.dtype, p.n_grid = _lib.PMPM_F32, 32
+    pin.n = 0
+    for fn in (L.pmpm_substep, L.pmpm_substep_begin, L.pmpm_substep_end):
+        rc = fn(C.byref(p), C.byref(pin), C.byref(pout), fake, 1 << 30, None)
+        assert rc == -1 and b"pmpm_workspace_init" in L.pmpm_last_error()
+    rc = L.pmpm_bin(None, _lib.PMPM_F32, 0, 32, 32.0, 1, None, None, None, fake, 1 << 30, None)
+    assert rc == -1 and b"pmpm_workspace_init" in L.pmpm_last_error()
+    q, c = C.c_int64(), C.c_int64()
+    rc = L.pmpm_last_step_stats(fake, 0, 32, 1, _lib.PMPM_F32, C.byref(q), C.byref(c), None)
+    assert rc == -1 and b"pmpm_workspace_init" in L.pmpm_last_error()
+    # a shape the workspace layout cannot take is refused by init itself
+    assert L.pmpm_workspace_init(fake, 1 << 30, 100, 2, 1, _lib.PMPM_F32, None) == -1
+    assert L.pmpm_workspace_init(fake, 1 << 30, -1, 32, 1, _lib.PMPM_F32, None) == -1
+
+
 def test_product_refuses_cpu_device():
     import torch
     from peel_mpm_amd import AnisoMPM
