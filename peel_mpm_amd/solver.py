@@ -75,6 +75,9 @@ class AnisoMPM:
         self._allow_any = False
         self._bc_key = None
         self._bc_id = None
+        self._p, self._p_key = None, None
+        self._pin = _lib.ParticlesIn()
+        self._out_n = -1
         _lib.lib()
 
     # ------------------------------------------------------------------ setup
@@ -205,62 +208,107 @@ class AnisoMPM:
             self._bc_id, self._bc_key = ids, key
         return self._bc_id, [b.value(time) for b in bcs]
 
-    def substep(self, time: float = 0.0):
-        L = _lib.lib()
+    def _config_key(self):
+        fc = None if self.f_clamp is None else (float(self.f_clamp[0]), float(self.f_clamp[1]))
+        return (self._code, self.n_grid, self.grid_lim, self.dx, self.inv_dx, self.dt, tuple(self._gravity_host),
+                self.grid_damp, fc, self.damage_every, self.stable_sort, self.exact_sigma_eig, self.damage_solver,
+                self.pcg_tol, self.pcg_max_iter)
+
+    def _step_params(self):
+        """The step's pmpm_step_params, built once per solver configuration;
+        the per-step fields are set by _prepare."""
+        key = self._config_key()
+        if key != self._p_key:
+            p = _lib.StepParams()
+            p.dtype, p.n_grid = self._code, self.n_grid
+            p.grid_lim, p.dx, p.inv_dx, p.dt = self.grid_lim, self.dx, self.inv_dx, self.dt
+            p.gravity[:] = self._gravity_host
+            p.grid_damp = self.grid_damp
+            if self.f_clamp is not None:
+                p.use_fclamp, p.fclamp_lo, p.fclamp_hi = 1, float(self.f_clamp[0]), float(self.f_clamp[1])
+            p.dt_damage = self.dt * self.damage_every
+            p.stable_sort = int(self.stable_sort)
+            p.exact_sigma_eig = int(self.exact_sigma_eig)
+            p.damage_solver = _lib.DAMAGE_PCG if self.damage_solver == "pcg" else _lib.DAMAGE_EXPLICIT
+            p.pcg_tol, p.pcg_max_iter = self.pcg_tol, self.pcg_max_iter
+            self._p, self._p_key = p, key
+        return self._p
+
+    def _outputs(self, do_damage):
+        """Fresh output tensors for one step (the step rebinds the state to
+        them, as the reference's step does, anisompm.py:793-802), carved out
+        of one allocation."""
+        parts, total = self._out_parts_dmg if do_damage else self._out_parts
+        flat = torch.empty(total, device=self.device, dtype=self.dtype)
+        return {k: flat[o:o + sz].view(shape) for k, shape, o, sz in parts}
+
+    def _layout_outputs(self):
+        """(name, shape, start, size) of each output in the one allocation,
+        each start 256-byte aligned in fp32."""
+        n = self.n
+        base = [("x", (n, 3)), ("v", (n, 3)), ("F", (n, 3, 3)), ("C", (n, 3, 3)), ("lap", (n,)),
+                ("tau_last", (n, 3, 3)), ("J_last", (n,))]
+        extra = [("d", (n,)), ("D_hist", (n,)), ("phi", (n,)), ("phi_shear", (n,))]
+
+        def lay(lst):
+            parts, o = [], 0
+            for k, shape in lst:
+                sz = 1
+                for d in shape:
+                    sz *= d
+                parts.append((k, shape, o, sz))
+                o += (sz + 63) // 64 * 64
+            return parts, max(o, 1)
+        self._out_parts, self._out_parts_dmg = lay(base), lay(base + extra)
+        self._out_n = n
+
+    def _prepare(self, time):
+        """Everything one substep hands to libpmpm: (params, particles in,
+        particles out, the new tensors, whether colliders run on the
+        device).  Runs the plain-callable particle_bc hooks."""
         bc_id, bc_vals = self._velocity_bcs(time)
         if bc_id is None:
             for bc in self.particle_bc:
                 bc(self, time)
-        n = self.n
-        ws = self._workspace()
         for k in _STATE + _FLOAT_PARAMS:
             self._tensor(k)
         allow = self._tensor("allow", torch.bool)
         imask = self._tensor("interface_mask", torch.bool)
         do_damage = self._any_allow() and (self._step % self.damage_every == 0)
 
-        p = _lib.StepParams()
-        p.dtype, p.n_grid = self._code, self.n_grid
-        p.grid_lim, p.dx, p.inv_dx, p.dt = self.grid_lim, self.dx, self.inv_dx, self.dt
-        p.gravity[:] = self._gravity_host
-        p.grid_damp = self.grid_damp
-        if self.f_clamp is not None:
-            p.use_fclamp, p.fclamp_lo, p.fclamp_hi = 1, float(self.f_clamp[0]), float(self.f_clamp[1])
+        p = self._step_params()
         p.do_damage = int(do_damage)
-        p.dt_damage = self.dt * self.damage_every
-        device_cols = all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders) and \
-            len(self.colliders) <= _lib.MAX_COLLIDERS
+        device_cols = len(self.colliders) <= _lib.MAX_COLLIDERS and \
+            all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders)
+        p.n_colliders = len(self.colliders) if device_cols else 0
         if device_cols:
-            p.n_colliders = len(self.colliders)
             for k, c in enumerate(self.colliders):
                 c.describe(time, p.colliders[k])
         p.export_active = int(not device_cols)
         p.n_vel_bc = len(bc_vals)
         for k, v in enumerate(bc_vals):
             p.vel_bc[k][:] = v
-        p.stable_sort = int(self.stable_sort)
-        p.exact_sigma_eig = int(self.exact_sigma_eig)
-        p.damage_solver = _lib.DAMAGE_PCG if self.damage_solver == "pcg" else _lib.DAMAGE_EXPLICIT
-        p.pcg_tol, p.pcg_max_iter = self.pcg_tol, self.pcg_max_iter
 
-        pin = _lib.ParticlesIn()
-        pin.n = n
+        pin = self._pin
+        pin.n = self.n
         for k in _STATE + _FLOAT_PARAMS:
             setattr(pin, k, getattr(self, k).data_ptr())
         pin.allow = allow.data_ptr()
         pin.interface_mask = imask.data_ptr()
         pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
 
-        new = {k: torch.empty_like(getattr(self, k)) for k in ("x", "v", "F", "C", "lap")}
-        new["tau_last"] = torch.empty(n, 3, 3, device=self.device, dtype=self.dtype)
-        new["J_last"] = torch.empty(n, device=self.device, dtype=self.dtype)
-        if do_damage:
-            for k in ("d", "D_hist", "phi", "phi_shear"):
-                new[k] = torch.empty(n, device=self.device, dtype=self.dtype)
+        if self._out_n != self.n:
+            self._layout_outputs()
+        new = self._outputs(do_damage)
         pout = _lib.ParticlesOut()
         for k, t in new.items():
             setattr(pout, k, t.data_ptr())
+        return p, pin, pout, new, device_cols
 
+    def substep(self, time: float = 0.0):
+        L = _lib.lib()
+        p, pin, pout, new, device_cols = self._prepare(time)
+        ws = self._workspace()
         st = _lib.stream_ptr(self.device)
         args = (C.byref(p), C.byref(pin), C.byref(pout), ws.data_ptr(), ws.numel(), st)
         if device_cols:
