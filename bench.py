@@ -2,25 +2,40 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
                     [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
-                    [--no-config2] [--no-carry] [--no-builders] [--no-tune]
+                    [--no-config2] [--no-carry] [--no-builders] [--tune | --launch-config JSON]
 
 Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
 (binning, stress/damage, P2G, grid update, G2P, advection) on particle state
 already resident in HBM; ``value`` = particles x steps / wall seconds of the
-K timed steps (bracketed by torch.cuda.synchronize()).
+K timed steps (bracketed by torch.cuda.synchronize()).  The headline runs
+the reference's explicit damage update (the parity mode); the config's
+implicit damage solve (PCG, tol 1e-6, <= 50 iterations) is the ``pcg`` leg,
+with its own rooflines.
 
-``roofline`` is for the dominant kernel, timed live with HIP events recorded
-by libpmpm on the launch stream (pmpm_timing): achieved = its algorithmic
+Algorithmic bytes follow one fixed convention (fp32):
+  whole step   289 B per particle (27 state values read and written, 18
+               material values and the allow byte read) + 32 B per active
+               quadratic node (m, mv written then read) + 16 B per active
+               cubic node (sum w d, sum w written then read)
+  stress+P2G   189 B per particle (state and materials read, d and D_hist
+               written) + 16 B per quadratic and 8 B per cubic node written
+  G2P          100 B per particle (x, v, F, C, lap written) + 16 B per
+               quadratic and 8 B per cubic node read
+  grid update  32 B per quadratic and 16 B per cubic node (its own read and
+               write; outside the step sum)
+  PCG          48 B per active cubic node + 16 B per particle per operator
+               application (the operator goes through the particles)
+  diagnostics  phi, phi_shear, tau_last, J_last (48 B per particle written
+               each step) are outside the convention and reported apart
+``roofline`` is for the dominant kernel group, timed live with HIP events
+recorded by libpmpm on the launch stream (pmpm_timing): its algorithmic
 bytes per launch / its mean launch time, against MI355X's 8 TB/s HBM peak.
-Algorithmic bytes per launch (fp32, damage on):
-  stress+P2G  238 B per particle + 24 B per active grid node
-  G2P         152 B per particle + 16 B per active grid node
-  grid update  40 B per active grid node
-  binning      32 B per particle
-Whole-step rooflines (``step_roofline``, and the batched form's) use
-289 B per particle + 32 B per active quadratic node + 16 B per active cubic
-node; forward+adjoint adds 834 B per particle-step for the backward pass
-(segment recompute + adjoint kernels).
+``traffic`` is the measured HBM bytes of that group (rocprofv3 --pmc,
+committed under profiles/), given only when the committed measurement was
+taken at the launch shape this run uses.  The plain step runs at a pinned
+launch shape (the autotuner's pick for this scene) unless --tune asks for a
+live search.  Forward+adjoint adds 834 B per particle-step for the backward
+pass (segment recompute + adjoint kernels).
 ``cpu_baseline`` times the oracle (an op-for-op torch-CPU restatement of the
 reference step, bit-identical to it on the CPU) on this host, on a bounded
 sample of the same scene.
@@ -41,21 +56,34 @@ sys.path.insert(0, ROOT)
 METRIC = "particle-steps/s at 1M fp32, batched vs plain; adjoint/forward time ratio"
 HBM_PEAK_GBS = 8000.0
 KERNELS = ("binning", "stress_p2g", "grid_update", "g2p")
-PCG_GROUP = 4          # pmpm_timing group of the implicit damage solve
+PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage solve, one CG iteration
 
+# The plain step's pinned launch shape: the autotuner's pick for config1
+# (autotune.tune_solver on this scene), and the shape profiles/ were taken at.
+PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 1,
+                 "g2p_threads": 256}
 
 STEP_B_PER_PARTICLE = 289
+DIAG_B_PER_PARTICLE = 48
 ADJ_B_PER_PARTICLE = 834
 
 
-def step_roofline(step_bytes, seconds_per_step):
-    gbs = step_bytes / seconds_per_step / 1e9
-    return {"algorithmic_bytes": step_bytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+def step_bytes(n, n_q, n_c):
+    return STEP_B_PER_PARTICLE * n + 32 * n_q + 16 * n_c
 
 
-def algorithmic_bytes(kernel, n, n_act):
-    return {"binning": 32 * n, "stress_p2g": 238 * n + 24 * n_act, "grid_update": 40 * n_act,
-            "g2p": 152 * n + 16 * n_act}[kernel]
+def pcg_apply_bytes(n, n_c):
+    return 48 * n_c + 16 * n
+
+
+def step_roofline(nbytes, seconds_per_step):
+    gbs = nbytes / seconds_per_step / 1e9
+    return {"algorithmic_bytes": nbytes, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
+def algorithmic_bytes(kernel, n, n_q, n_c):
+    return {"binning": 20 * n, "stress_p2g": 189 * n + 16 * n_q + 8 * n_c, "grid_update": 32 * n_q + 16 * n_c,
+            "g2p": 100 * n + 16 * n_q + 8 * n_c}[kernel]
 
 
 def make_scene(name):
@@ -63,10 +91,16 @@ def make_scene(name):
     return {"config0": scenes.config0, "config1": scenes.config1, "config2": scenes.config2}[name]()
 
 
-def time_gpu(spec, steps, warmup, tune=True):
+def time_gpu(spec, steps, warmup, tune=True, prep=None, **solver_kw):
+    """Time ``steps`` substeps of the scene (after ``warmup``), then the same
+    number again with live per-group timing.  Returns the solver, wall
+    seconds, ms per launch and launches of each timing group, active nodes,
+    and the launch shape used."""
     import torch
     from peel_mpm_amd import AnisoMPM, _lib, autotune, scenes
-    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", **solver_kw)
+    if prep:
+        prep(sim)
     L = _lib.lib()
     t_tune = time.perf_counter()
     launch = autotune.tune_solver(sim, steps=10) if tune else autotune.get_config()
@@ -79,19 +113,19 @@ def time_gpu(spec, steps, warmup, tune=True):
         sim.substep((warmup + s) * spec["dt"])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-kernel live timing over a second pass of the same number of steps
+    # per-group live timing over a second pass of the same number of steps
     L.pmpm_timing(1)
     for s in range(steps):
         sim.substep((warmup + steps + s) * spec["dt"])
     torch.cuda.synchronize()
-    ms = (C.c_double * 4)()
-    cnt = (C.c_int64 * 4)()
-    _lib.check(L.pmpm_timing_read(ms, cnt, 4), "pmpm_timing_read")
+    ms = (C.c_double * 6)()
+    cnt = (C.c_int64 * 6)()
+    _lib.check(L.pmpm_timing_read(ms, cnt, 6), "pmpm_timing_read")
     L.pmpm_timing(0)
     n_q, n_c = sim.last_step_stats()
-    finite = bool(torch.isfinite(sim.x).all())
-    per_kernel = {k: (ms[i] / max(cnt[i], 1)) for i, k in enumerate(KERNELS)}
-    return sim.n, wall, per_kernel, (n_q, n_c), finite, {"config": launch, "tuning_s": round(t_tune, 2)}
+    groups = {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS + ("pcg_solve", "pcg_iteration"))}
+    return sim, wall, groups, (n_q, n_c), {"config": launch, "source": "autotuned" if tune else "pinned",
+                                           "tuning_s": round(t_tune, 2)}
 
 
 def time_batched(n_scenes, steps, warmup):
@@ -145,38 +179,55 @@ def time_batched(n_scenes, steps, warmup):
 
 
 def time_adjoint(steps, every):
-    """The differentiable form: config4 (32,768 particles), a forward rollout
-    of ``steps`` substeps vs its backward pass (segment recompute from the
-    checkpoints every ``every`` steps + the adjoint steps)."""
+    """The differentiable form, config4 as specified: the config-0 scene at
+    32,768 particles, ``steps`` substeps; the loss is the squared distance of
+    the final positions to a target made by the same forward pass with
+    E = 1.3e5, nu = 0.28 and the fibres turned 15 degrees about z;
+    gradients with respect to per-particle E, nu, fibre direction and the
+    initial velocity.  Times the forward rollout (checkpoint every
+    ``every``) against its backward pass (segment recompute + adjoint
+    steps)."""
+    import math
+
     import torch
     from peel_mpm_amd import AnisoMPM, scenes
     from peel_mpm_amd.adjoint import material_tensors, rollout
     spec = scenes.config4(dtype=torch.float32)
     sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0")
-    n = sim.n
-    E = torch.full((n,), 1e5, device="cuda:0", requires_grad=True)
-    nu = torch.full((n,), 0.3, device="cuda:0", requires_grad=True)
+    n, dev = sim.n, sim.device
+    c, s_ = math.cos(math.radians(15.0)), math.sin(math.radians(15.0))
+    Rz = torch.tensor([[c, -s_, 0.0], [s_, c, 0.0], [0.0, 0.0, 1.0]], device=dev)
+    with torch.no_grad():
+        mu_t, lam_t, A0_t = material_tensors(torch.full((n,), 1.3e5, device=dev), torch.full((n,), 0.28, device=dev),
+                                             sim.a0 @ Rz.T, sim.alpha)
+        target = rollout(sim, sim.v, mu_t, lam_t, A0_t, steps, every=every).clone()
+    E = torch.full((n,), 1e5, device=dev, requires_grad=True)
+    nu = torch.full((n,), 0.3, device=dev, requires_grad=True)
     a0 = sim.a0.clone().requires_grad_(True)
     v0 = sim.v.clone().requires_grad_(True)
-    target = sim.x.detach() + 0.01
-    res = {}
+    res, loss = {}, None
     for rep in range(2):                   # first pass warms up allocations and code objects
+        for t in (E, nu, a0, v0):
+            t.grad = None
         mu, lam, A0 = material_tensors(E, nu, a0, sim.alpha)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         xT = rollout(sim, v0, mu, lam, A0, steps, every=every)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        ((xT - target) ** 2).sum().backward()
+        loss = ((xT - target) ** 2).sum()
+        loss.backward()
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         res = {"fwd": t1 - t0, "bwd": t2 - t1}
     finite = all(bool(torch.isfinite(t.grad).all()) for t in (E, nu, a0, v0))
     n_q, n_c = sim.last_step_stats()
-    fwd_bytes = (STEP_B_PER_PARTICLE * n + 32 * n_q + 16 * n_c) * steps
+    fwd_bytes = step_bytes(n, n_q, n_c) * steps
     bwd_bytes = ADJ_B_PER_PARTICLE * n * steps
     tot_s = res["fwd"] + res["bwd"]
-    return {"workload": f"config4: {n} particles, 64^3 grid, {steps} steps, checkpoint every {every}",
+    return {"workload": f"config4: {n} particles, 64^3 grid, {steps} steps, checkpoint every {every}; target from "
+                        "E 1.3e5, nu 0.28, fibres turned 15 deg about z; gradients w.r.t. E, nu, fibre, v0",
+            "loss": float(loss),
             "roofline": {"bound": "hbm", "algorithmic_bytes": fwd_bytes + bwd_bytes,
                          "achieved_GBs": (fwd_bytes + bwd_bytes) / tot_s / 1e9, "peak": HBM_PEAK_GBS,
                          "frac": (fwd_bytes + bwd_bytes) / tot_s / 1e9 / HBM_PEAK_GBS,
@@ -187,34 +238,41 @@ def time_adjoint(steps, every):
             "grads_finite": finite}
 
 
-def time_pcg(spec, steps, warmup):
-    """The implicit (PCG) damage mode on the same scene: whole-step rate and
-    the solve's own time per step (HIP events around its kernels)."""
+def time_pcg(spec, steps, warmup, label, prep=None):
+    """The implicit (PCG) damage mode on ``spec``: whole-step rate and
+    rooflines of the step, of the solve and of one CG iteration (HIP events
+    around the solve's kernels and around each iteration's)."""
     import torch
-    from peel_mpm_amd import AnisoMPM, _lib, scenes
-    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", damage_solver="pcg", pcg_tol=1e-6, pcg_max_iter=50)
-    L = _lib.lib()
-    for s in range(warmup):
-        sim.substep(s * spec["dt"])
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(steps):
-        sim.substep((warmup + s) * spec["dt"])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    sim, wall, groups, (n_q, n_c), launch = time_gpu(spec, steps, warmup, tune=False, prep=prep, damage_solver="pcg",
+                                                     pcg_tol=1e-6, pcg_max_iter=50)
     it, rel = sim.last_damage_solve()
-    L.pmpm_timing(1)
-    for s in range(steps):
-        sim.substep((warmup + steps + s) * spec["dt"])
-    torch.cuda.synchronize()
-    ms = (C.c_double * 5)()
-    cnt = (C.c_int64 * 5)()
-    _lib.check(L.pmpm_timing_read(ms, cnt, 5), "pmpm_timing_read")
-    L.pmpm_timing(0)
-    return {"workload": "same scene, damage_solver='pcg' (tol 1e-6 on the preconditioned residual, <= 50 iterations)",
-            "value": sim.n * steps / wall, "unit": "particle-steps/s", "ms_per_step": wall / steps * 1e3,
-            "solve_ms_per_step": ms[PCG_GROUP] / max(cnt[PCG_GROUP], 1), "last_iterations": it,
-            "last_residual": rel, "finite": bool(torch.isfinite(sim.d).all())}
+    n = sim.n
+    solve_ms, solves = groups["pcg_solve"]
+    iter_ms, iters = groups["pcg_iteration"]
+    it_per_step = iters / max(solves, 1)
+    per_iter_ms = iter_ms / max(iters, 1)
+    applies = it_per_step + 1                       # the initial residual's operator application
+    ab = pcg_apply_bytes(n, n_c)
+    sb = step_bytes(n, n_q, n_c) + applies * ab
+    ms_step = wall / steps * 1e3
+    return {"workload": f"{label}: {n} particles, damage_solver='pcg' (tol 1e-6 on the preconditioned residual, "
+                        "<= 50 iterations)",
+            "value": n * steps / wall, "unit": "particle-steps/s", "ms_per_step": ms_step,
+            "solve_ms_per_step": solve_ms / max(solves, 1), "iterations_per_step": it_per_step,
+            "ms_per_iteration": per_iter_ms, "bytes_per_iteration": ab,
+            "last_iterations": it, "last_residual": rel,
+            "step_roofline": dict(step_roofline(sb, ms_step * 1e-3), peak=HBM_PEAK_GBS),
+            "iteration_roofline": dict(step_roofline(ab, per_iter_ms * 1e-3), peak=HBM_PEAK_GBS),
+            "active_nodes": {"quadratic": n_q, "cubic": n_c},
+            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in groups.items()},
+            "finite": bool(torch.isfinite(sim.d).all())}
+
+
+def stiff_damage(sim):
+    """A stiffer damage system on the same scene: eta 1e-4 (c = dt/eta =
+    0.5) and l0 = 2 dx, so the solve takes many CG iterations."""
+    sim.eta.fill_(1e-4)
+    sim.l0.fill_(2.0 * sim.dx)
 
 
 CARRY_B_PER_SPLAT = 180    # bind 8, x 12, F 36, d 4, off0 12, cov0 36, color 12 in; pos 12, cov 36, color 12 out
@@ -408,17 +466,23 @@ def copy_calibration(mib):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_config1.md")
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of ``kernel`` on ``workload`` from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled);
-    None when no summary covers them."""
+def measured_traffic(workload, kernel, launch):
+    """HBM bytes per launch of kernel group ``kernel`` on ``workload`` from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE,
+    copy-kernel scaled), and the launch shape it was measured at; the bytes
+    are None when the summary does not cover the group or was taken at
+    another launch shape than ``launch``."""
     try:
         text = open(TRAFFIC_PROFILE).read()
     except OSError:
-        return None
+        return None, None
+    m = re.search(r"^launch_config\s+(\{.*\})\s*$", text, re.M)
+    shape = json.loads(m.group(1)) if m else None
     m = re.search(r"^traffic_bytes_per_launch\s+" + re.escape(workload) + r"\s+" + re.escape(kernel) + r"\s+(\d+)",
                   text, re.M)
-    return int(m.group(1)) if m else None
+    if not m or shape != launch:
+        return None, shape
+    return int(m.group(1)), shape
 
 
 def time_cpu(spec, steps):
@@ -452,10 +516,11 @@ def main():
     ap.add_argument("--no-pcg", action="store_true")
     ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry and peel-observable legs")
     ap.add_argument("--no-builders", action="store_true", help="skip the scene-builder and Option A legs")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap.add_argument("--tune", action="store_true",
+                    help="autotune the launch shape live instead of the pinned one (the batched form always tunes)")
     ap.add_argument("--no-config2", action="store_true", help="skip the 8M-particle config2 scene")
     ap.add_argument("--launch-config", default=None, metavar="JSON",
-                    help="fixed launch shape, e.g. '{\"bucket_wgs\": 4096}' (implies --no-tune)")
+                    help="fixed launch shape, e.g. '{\"bucket_wgs\": 4096}' (default: the pinned shape)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="after the plain run, copy a MIB-MiB buffer (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -464,37 +529,45 @@ def main():
               "scenes shard across GPUs as independent processes", file=sys.stderr)
         sys.exit(2)
 
-    if args.launch_config:
-        from peel_mpm_amd import autotune
-        autotune.set_config(json.loads(args.launch_config))
-        args.no_tune = True
+    from peel_mpm_amd import autotune
+    tune = args.tune
+    if not tune:
+        autotune.set_config(json.loads(args.launch_config) if args.launch_config else PINNED_LAUNCH)
     spec = make_scene(args.workload)
-    n, wall, per_kernel, (n_q, n_c), finite, tuned = time_gpu(spec, args.steps, args.warmup, tune=not args.no_tune)
+    sim, wall, groups, (n_q, n_c), launch = time_gpu(spec, args.steps, args.warmup, tune=tune)
+    n = sim.n
+    finite = bool(__import__("torch").isfinite(sim.x).all())
+    del sim
+    per_kernel = {k: groups[k][0] / max(groups[k][1], 1) for k in KERNELS}
     value = n * args.steps / wall
     dominant = max(per_kernel, key=per_kernel.get)
-    n_act = max(n_q, n_c)
-    alg = algorithmic_bytes(dominant, n, n_act)
+    alg = algorithmic_bytes(dominant, n, n_q, n_c)
     achieved = alg / (per_kernel[dominant] * 1e-3) / 1e9
-    step_bytes = STEP_B_PER_PARTICLE * n + 32 * n_q + 16 * n_c
+    traffic, traffic_shape = measured_traffic(args.workload, dominant, launch["config"])
     out = {
         "metric": METRIC, "value": value, "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{args.workload}: {n} particles, {spec['n_grid']}^3 grid, dt={spec['dt']}",
-                   "particles": n, "n_grid": spec["n_grid"], "dt": spec["dt"], "form": "plain single-scene"},
+        "config": {"workload": f"{args.workload}: {n} particles, {spec['n_grid']}^3 grid, dt={spec['dt']}, explicit "
+                               "damage update (the reference's; the config's PCG damage solve is the pcg leg)",
+                   "particles": n, "n_grid": spec["n_grid"], "dt": spec["dt"], "form": "plain single-scene",
+                   "damage": "explicit"},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload, dominant),
-                     "algorithmic_bytes": alg},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg,
+                     "launch_ms": per_kernel[dominant],
+                     "diagnostics_bytes_excluded": DIAG_B_PER_PARTICLE * n,
+                     "traffic_launch_config": traffic_shape},
         "kernel_ms": {k: round(v, 5) for k, v in per_kernel.items()},
-        "step_roofline": step_roofline(step_bytes, wall / args.steps),
+        "step_roofline": step_roofline(step_bytes(n, n_q, n_c), wall / args.steps),
         "active_nodes": {"quadratic": n_q, "cubic": n_c},
-        "autotune": tuned,
+        "launch": launch,
         "finite": finite,
     }
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_pcg:
-        out["pcg"] = time_pcg(spec, min(args.steps, 20), 2)
+        out["pcg"] = time_pcg(spec, min(args.steps, 20), 2, args.workload)
+        out["pcg_stiff"] = time_pcg(spec, 10, 1, f"{args.workload}, eta 1e-4 and l0 = 2 dx", prep=stiff_damage)
     if not args.no_carry:
         out["gaussian_carry"] = time_carry(spec, 4, 20)
         out["peel_observables"] = time_observe(spec, 50)
@@ -504,12 +577,15 @@ def main():
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")
-        n2, wall2, pk2, (q2, c2), fin2, tuned2 = time_gpu(s2, 10, 2, tune=not args.no_tune)
-        out["config2"] = {"workload": f"config2: {n2} particles, 256^3 grid, two slabs, dt=2e-05", "value": n2 * 10 / wall2,
-                          "unit": "particle-steps/s", "ms_per_step": wall2 / 10 * 1e3,
-                          "kernel_ms": {k: round(v, 5) for k, v in pk2.items()},
-                          "active_nodes": {"quadratic": q2, "cubic": c2}, "finite": fin2, "autotune": tuned2}
-        del s2
+        sim2, wall2, g2, (q2, c2), tuned2 = time_gpu(s2, 10, 2, tune=tune)
+        n2 = sim2.n
+        out["config2"] = {"workload": f"config2: {n2} particles, 256^3 grid, two slabs, dt=2e-05, explicit damage",
+                          "value": n2 * 10 / wall2, "unit": "particle-steps/s", "ms_per_step": wall2 / 10 * 1e3,
+                          "kernel_ms": {k: round(g2[k][0] / max(g2[k][1], 1), 5) for k in KERNELS},
+                          "step_roofline": step_roofline(step_bytes(n2, q2, c2), wall2 / 10),
+                          "active_nodes": {"quadratic": q2, "cubic": c2},
+                          "finite": bool(torch.isfinite(sim2.x).all()), "launch": tuned2}
+        del s2, sim2
         torch.cuda.empty_cache()
     if not args.no_batched:
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)

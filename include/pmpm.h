@@ -263,9 +263,10 @@ int pmpm_sym_eig(const void *A, int32_t dtype, int64_t n, void *evals, void *V, 
 
 /* Live kernel timing with HIP events on the launch stream (for benchmarks):
  * pmpm_timing(1) clears and starts recording around each kernel group
- * (0 binning, 1 stress+P2G, 2 grid update, 3 G2P); pmpm_timing_read
- * synchronises and returns each group's summed milliseconds and launch
- * count (n <= 4 groups).  Adds two event records per group per step. */
+ * (0 binning, 1 stress+P2G, 2 grid update, 3 G2P, 4 the PCG damage solve,
+ * 5 one CG iteration within it); pmpm_timing_read synchronises and returns
+ * each group's summed milliseconds and launch count (n <= 6 groups).  Adds
+ * two event records per group per step. */
 int pmpm_timing(int32_t enable);
 int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
 

@@ -77,8 +77,9 @@ inline int nblk(int ng) { return (ng + BLK - 1) / BLK; }
 
 // Optional live timing: HIP events recorded on the launch stream around each
 // kernel group, summed on read.  Groups: 0 binning, 1 p2g, 2 grid, 3 g2p,
-// 4 the PCG damage solve (drive, assembly, iterations, interpolation).
-constexpr int NTIMED = 5;
+// 4 the PCG damage solve (drive, assembly, iterations, interpolation),
+// 5 one CG iteration of it (operator, fold, reductions, update).
+constexpr int NTIMED = 6;
 struct Timing {
     bool on = false;
     std::vector<hipEvent_t> ev[NTIMED];   // start/stop pairs
@@ -840,11 +841,12 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     A.dsrt[i] = d;
 }
 
-// One particle's P2G data in its cell's 4x4x4 frame: weights per axis and
-// frame node (quadratic zero-padded, cubic), node offsets dq (x_i - x_p per
-// axis), affine matrix, mass * v, mass and d.
+// One particle's P2G data in its cell's 4x4x4 frame: per axis and frame
+// node the quadratic weight W (zero-padded to the frame), D = W (x_i - x_p)
+// for that axis, and the cubic weight wc; the affine matrix, mass * v, mass
+// and d.
 template <typename T> struct P2GPart {
-    T wq[4][3], wc[4][3], dq[4][3], af[3][3], mv[3], mass, d;
+    T W[4][3], D[4][3], wc[4][3], af[3][3], mv[3], mass, d;
     int bcb[3];
 };
 
@@ -870,16 +872,30 @@ __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int 
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             const int j = k - sh[a];
-            q.wq[k][a] = j == 0 ? w[0][a] : (j == 1 ? w[1][a] : (j == 2 ? w[2][a] : T(0)));
-            q.dq[k][a] = (T(j) - fx[a]) * A.dx;
+            q.W[k][a] = j == 0 ? w[0][a] : (j == 1 ? w[1][a] : (j == 2 ? w[2][a] : T(0)));
+            q.D[k][a] = q.W[k][a] * ((T(j) - fx[a]) * A.dx);
         }
     }
 }
 
+// Zero contribution (a pair's second slot when it has no partner).
+template <typename T> __device__ __forceinline__ void p2g_mute(P2GPart<T>& q) {
+    q.mass = T(0);
+    q.d = T(0);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { q.W[k][a] = T(0); q.D[k][a] = T(0); q.wc[k][a] = T(0); }
+}
+
 // Scatter NP particles that share one frame (lp: arena position of each
-// frame node per axis) into one wavefront's arena: per node the NP
-// contributions are summed in registers, then one read-modify-write of the
-// node's (m, mv) and (sum w d, sum w).  Lanes own distinct cells, so for a
+// frame node per axis) into one wavefront's arena.  A particle adds to node
+// (i, j, k) of its frame
+//   m Wi Wj Wk,   Wi Wj Wk mv + A (Di Wj Wk, Wi Dj Wk, Wi Wj Dk)   (momentum)
+//   d wci wcj wck, wci wcj wck                                       (cubic)
+// which separates: per (i, j) the row terms P = Wi Wj mv + A0 Di Wj + A1 Wi Dj
+// and Q = A2 Wi Wj, then per k momentum += P Wk + Q Dk -- fused multiply-adds
+// straight into the node's arena values.  Lanes own distinct cells, so for a
 // fixed node offset the 64 lanes touch 64 distinct entries and no atomics
 // are needed.  Different nodes of different lanes can be one entry, so the
 // accesses must reach the LDS in program order, which executes one
@@ -899,41 +915,38 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ox = 0; ox < 4; ++ox) {
-        T bx[NP][3];
-#pragma unroll
-        for (int j = 0; j < NP; ++j)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) bx[j][c] = q[j].mv[c] + q[j].af[c][0] * q[j].dq[ox][0];
 #pragma unroll
         for (int oy = 0; oy < 4; ++oy) {
-            T wqxy[NP], wcxy[NP], bxy[NP][3];
+            T P[NP][3], Q[NP][3], M[NP], CW[NP], CD[NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
-                wqxy[j] = q[j].wq[ox][0] * q[j].wq[oy][1];
-                wcxy[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
+                const T WW = q[j].W[ox][0] * q[j].W[oy][1];
+                const T DW = q[j].D[ox][0] * q[j].W[oy][1];
+                const T WD = q[j].W[ox][0] * q[j].D[oy][1];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) bxy[j][c] = bx[j][c] + q[j].af[c][1] * q[j].dq[oy][1];
+                for (int c = 0; c < 3; ++c) {
+                    P[j][c] = q[j].af[c][1] * WD + (q[j].af[c][0] * DW + WW * q[j].mv[c]);
+                    Q[j][c] = q[j].af[c][2] * WW;
+                }
+                M[j] = WW * q[j].mass;
+                CW[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
+                CD[j] = CW[j] * q[j].d;
             }
             const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
-                T a0, a1, a2, a3, e0, e1;
 #pragma unroll
                 for (int j = 0; j < NP; ++j) {
-                    const T wk = wqxy[j] * q[j].wq[oz][2];
-                    const T wck = wcxy[j] * q[j].wc[oz][2];
-                    const T t0 = wk * q[j].mass;
-                    const T t1 = wk * (bxy[j][0] + q[j].af[0][2] * q[j].dq[oz][2]);
-                    const T t2 = wk * (bxy[j][1] + q[j].af[1][2] * q[j].dq[oz][2]);
-                    const T t3 = wk * (bxy[j][2] + q[j].af[2][2] * q[j].dq[oz][2]);
-                    const T t4 = wck * q[j].d;
-                    if (j == 0) { a0 = t0; a1 = t1; a2 = t2; a3 = t3; e0 = t4; e1 = wck; }
-                    else { a0 += t0; a1 += t1; a2 += t2; a3 += t3; e0 += t4; e1 += wck; }
+                    const T Wk = q[j].W[oz][2], Dk = q[j].D[oz][2], Ck = q[j].wc[oz][2];
+                    c0 = M[j] * Wk + c0;
+                    c1 = P[j][0] * Wk + (Q[j][0] * Dk + c1);
+                    c2 = P[j][1] * Wk + (Q[j][1] * Dk + c2);
+                    c3 = P[j][2] * Wk + (Q[j][2] * Dk + c3);
+                    f0 = CD[j] * Ck + f0;
+                    f1 = CW[j] * Ck + f1;
                 }
                 const int l = lxy + lp[2][oz];
-                c0 += a0; c1 += a1; c2 += a2; c3 += a3;
                 sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3;
-                f0 += e0; f1 += e1;
                 sc[l][0] = f0; sc[l][1] = f1;
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
@@ -1011,13 +1024,7 @@ k_p2g(StepArgs<T> A) {
             // particles of one cell share the frame unless they lie beyond the
             // grid (cells clamped together); such a second particle goes alone
             const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
-            if (!same) {
-                q[1].mass = T(0); q[1].d = T(0);
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) { q[1].wq[k][a] = T(0); q[1].wc[k][a] = T(0); }
-            }
+            if (!same) p2g_mute(q[1]);
             p2g_scatter<T, 2>(s_q[wsel], s_c[wsel], lp, q);
             if (two && !same) {
                 P2GPart<T> q1[1];
@@ -2316,6 +2323,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         while (issued < p->pcg_max_iter) {
             const int k_end = capturing ? p->pcg_max_iter : std::min(p->pcg_max_iter, issued + chunk);
             for (; issued < k_end; ++issued) {
+                TimedScope ti(5, st);
                 hipLaunchKernelGGL((k_pcg_apply<T, false>), PG, PB, 0, st, A);
                 hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY>), GG, GB, 0, st, A);
                 hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_ALPHA>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts,
