@@ -169,6 +169,24 @@ int pmpm_substep_begin(const pmpm_step_params *p, const pmpm_particles_in *in,
                        const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
 int pmpm_substep_end(const pmpm_step_params *p, const pmpm_particles_in *in,
                      const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
+/* No counterpart in the reference.  pmpm_substep without a workspace
+ * argument and without allocating, for callers that cannot call hipMalloc
+ * or an allocating library routine: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation.  The workspace is
+ * PMPM_NOALLOC_SCRATCH_BYTES (64 MiB) of device memory the library itself
+ * holds, a static device array that every GPU the library runs on keeps
+ * until the process ends -- enough for a 64^3-grid scene of up to about
+ * 250,000 fp32 particles (pmpm_workspace_bytes(n, n_grid, n_scenes, dtype)
+ * is the need; a larger step returns PMPM_ESIZE before anything is
+ * enqueued).  Every call on a GPU shares that GPU's scratch and leaves its
+ * contents behind; the call clears what must start at zero itself.  Calls
+ * that overlap on one GPU (two streams or two threads) race.  The PCG
+ * damage mode enqueues all pcg_max_iter iterations (no host polling);
+ * keep_for_adjoint is refused (PMPM_EARG): the adjoint takes a caller-owned
+ * workspace. */
+#define PMPM_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int pmpm_substep_noalloc(const pmpm_step_params *p, const pmpm_particles_in *in,
+                         const pmpm_particles_out *out, void *stream);
 /* Byte offsets into ws of the grid velocity (n_grid^3 x 4: vx,vy,vz,gd) and
  * of the dense active-node mask (n_grid^3 uint8, valid after begin with
  * export_active=1), for a host that resolves colliders itself. */

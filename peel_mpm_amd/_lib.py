@@ -14,6 +14,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpmpm.so")
 
 PMPM_F32, PMPM_F64 = 0, 1
+NOALLOC_SCRATCH_BYTES = 64 << 20
 MAX_COLLIDERS, MAX_VEL_BC = 8, 8
 SPHERE, HALFSPACE = 1, 2
 STICKY, SLIP = 0, 1
@@ -26,7 +27,7 @@ EXPORTS = ("pmpm_workspace_bytes", "pmpm_workspace_init", "pmpm_substep", "pmpm_
            "pmpm_set_launch_config", "pmpm_get_launch_config", "pmpm_gaussian_carry", "pmpm_observe_workspace_bytes",
            "pmpm_observe_frame", "pmpm_compact_workspace_bytes", "pmpm_ball_particles", "pmpm_grid_block",
            "pmpm_fill_workspace_bytes", "pmpm_fill_bounds", "pmpm_fill_interior", "pmpm_directional_pk1",
-           "pmpm_directional_kirchhoff")
+           "pmpm_directional_kirchhoff", "pmpm_substep_noalloc")
 
 
 class Pcg64(C.Structure):
@@ -103,6 +104,7 @@ def lib():
     L.pmpm_workspace_init.argtypes = [vp, sz, i64, i32, i32, i32, vp]
     for f in (L.pmpm_substep, L.pmpm_substep_begin, L.pmpm_substep_end):
         f.argtypes = [C.POINTER(StepParams), C.POINTER(ParticlesIn), C.POINTER(ParticlesOut), vp, sz, vp]
+    L.pmpm_substep_noalloc.argtypes = [C.POINTER(StepParams), C.POINTER(ParticlesIn), C.POINTER(ParticlesOut), vp]
     L.pmpm_grid_views.argtypes = [i64, i32, i32, i32, C.POINTER(sz), C.POINTER(sz)]
     L.pmpm_last_step_stats.argtypes = [vp, i64, i32, i32, i32, C.POINTER(i64), C.POINTER(i64), vp]
     L.pmpm_bin.argtypes = [vp, i32, i64, i32, dbl, i32, vp, vp, vp, vp, sz, vp]

@@ -2546,6 +2546,71 @@ int pmpm_substep(const pmpm_step_params* p, const pmpm_particles_in* in, const p
     return pmpm_substep_end(p, in, out, ws, ws_bytes, stream);
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- no-allocation step
+// The scratch of pmpm_substep_noalloc: the step's workspace, in device
+// memory the library itself holds (every GPU the library runs on holds its
+// own copy until the process ends).  Host code never passes this name to a
+// kernel or a copy: it stands in for the array on the host.  The device
+// address comes from hipGetSymbolAddress, once per device, below.
+static __device__ __align__(256) unsigned char scratch[PMPM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the current device's scratch, looked up on the first
+// call on that device (with the device current) and kept.
+int scratch_address(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(PMPM_EARG, "%sdevice ordinal out of range: %lld", "", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr) return fail(PMPM_EHIP, "%sno device address for the scratch array", "");
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return PMPM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int pmpm_substep_noalloc(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out,
+                         void* stream) {
+    if (!p || !in || !out) return fail(PMPM_EARG, "%snull argument", "");
+    if (p->dtype != PMPM_F32 && p->dtype != PMPM_F64) return fail(PMPM_EARG, "%sunknown dtype %lld", "", p->dtype);
+    if (p->n_grid < 4 || p->n_grid > 1024) return fail(PMPM_EARG, "%sn_grid out of range: %lld", "", p->n_grid);
+    if (in->n < 0 || in->n > 0x7fffffffLL) return fail(PMPM_EARG, "%sparticle count out of range: %lld", "", in->n);
+    if (p->keep_for_adjoint) return fail(PMPM_EARG, "%skeep_for_adjoint needs a caller-owned workspace", "");
+    const Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
+    if (L.total > (size_t)PMPM_NOALLOC_SCRATCH_BYTES)
+        return fail(PMPM_ESIZE, "%sthis step needs %lld bytes of scratch; pmpm_substep_noalloc holds "
+                    "PMPM_NOALLOC_SCRATCH_BYTES (use pmpm_substep with a workspace)", "", (long long)L.total);
+    int64_t n;
+    if (int rc = check_step(p, in, out, nullptr, L.total, n)) return rc;
+    void* ws = nullptr;
+    if (int rc = scratch_address(&ws)) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // the previous call may have laid the scratch out for another shape:
+    // clear the zero-state sections with a fill kernel
+    const long long nz = (long long)L.zero_end();
+    hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks((nz / 16) < 1 ? 1 : nz / 16)), dim3(WG), 0, st, (unsigned char*)ws, nz);
+    HIPCHK(hipGetLastError());
+    t_untimed = true;
+    int rc = p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false)
+                                  : step_begin<float>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false);
+    if (!rc)
+        rc = p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, st) : step_end<float>(p, in, out, (char*)ws, st);
+    t_untimed = false;
+    return rc;
+}
+
 int pmpm_timing(int32_t enable) {
     g_timing.reset();
     g_timing.on = enable != 0;
