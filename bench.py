@@ -275,6 +275,40 @@ def stiff_damage(sim):
     sim.l0.fill_(2.0 * sim.dx)
 
 
+def time_small_frame(steps):
+    """The host floor of a small scene: config4's scene (32,768 particles,
+    fp32) stepped by run_frame eagerly (one substep call per step: argument
+    set-up plus ten kernel launches from the host) and replayed from a
+    captured graph (use_graph=True: one replay per frame), at the launch
+    shape the autotuner picks for it.  Also the host time one eager
+    substep call takes to enqueue its work."""
+    import torch
+    from peel_mpm_amd import AnisoMPM, autotune, scenes
+    spec = scenes.config4(dtype=torch.float32)
+    out = {"workload": f"config4 scene forward, {steps} substeps per frame, fp32"}
+    for graph in (False, True):
+        sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", use_graph=graph)
+        launch = autotune.tune_solver(sim, steps=10)
+        sim.run_frame(steps, 0.0)                     # warm-up (and capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run_frame(steps, steps * spec["dt"])
+        torch.cuda.synchronize()
+        per = (time.perf_counter() - t0) / steps
+        key = "graph" if graph else "eager"
+        out[f"{key}_us_per_step"] = per * 1e6
+        out[f"{key}_particle_steps_per_s"] = sim.n * steps / (per * steps)
+        out["launch"] = launch
+        if not graph:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for s in range(20):
+                sim.substep(s * spec["dt"])
+            out["eager_host_us_per_call"] = (time.perf_counter() - t0) / 20 * 1e6
+            torch.cuda.synchronize()
+    return out
+
+
 CARRY_B_PER_SPLAT = 180    # bind 8, x 12, F 36, d 4, off0 12, cov0 36, color 12 in; pos 12, cov 36, color 12 out
 
 
@@ -591,6 +625,7 @@ def main():
         out["batched"] = time_batched(args.batched_scenes, args.batched_steps, 2)
     if not args.no_adjoint:
         out["adjoint"] = time_adjoint(args.adjoint_steps, 10)
+        out["small_scene_frame"] = time_small_frame(100)
     if not args.no_cpu_baseline and args.cpu_steps > 0:
         cv, cores, cwall = time_cpu(spec, args.cpu_steps)
         out["cpu_baseline"] = {"value": cv, "unit": "particle-steps/s", "cores": cores, "kind": "port",

@@ -133,17 +133,19 @@ def describe_boundaries(colliders, vel_bcs, time: float, table: _lib.BoundaryTab
 
 class DeviceBoundaryTable:
     """The step's colliders and BC velocities in device memory
-    (``StepParams.boundary_table``), rewritten before every step, so a step
+    (``StepParams.boundary_table``), rewritten before every step (or before
+    every replay of a captured frame, one table per substep), so a step
     replayed from a HIP graph follows moving colliders and time-dependent
     velocities.  The host side is a ring of pinned buffers: a slot is reused
     only after the copy out of it has run (its event), so the host never
     overwrites values a pending copy still has to read."""
 
-    def __init__(self, device, slots: int = 4):
-        nb = C.sizeof(_lib.BoundaryTable)
+    def __init__(self, device, count: int = 1, slots: int = 4):
+        self.nbytes = C.sizeof(_lib.BoundaryTable)
+        self.count = int(count)
         self.device = torch.device(device)
-        self.dev = torch.zeros(nb, dtype=torch.uint8, device=self.device)
-        self._host = [torch.zeros(nb, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
+        self.dev = torch.zeros(self.nbytes * self.count, dtype=torch.uint8, device=self.device)
+        self._host = [torch.zeros(self.nbytes * self.count, dtype=torch.uint8, pin_memory=True) for _ in range(slots)]
         self._events = [None] * slots
         self._i = 0
         self.table = _lib.BoundaryTable()
@@ -152,15 +154,28 @@ class DeviceBoundaryTable:
     def ptr(self) -> int:
         return self.dev.data_ptr()
 
+    def entry(self, k: int) -> int:
+        """Device address of table k."""
+        return self.dev.data_ptr() + k * self.nbytes
+
     def write(self, colliders, vel_bcs, time: float):
         """Enqueue (on the current stream) the copy of the values at ``time``."""
-        describe_boundaries(colliders, vel_bcs, time, self.table)
+        self.write_many(colliders, vel_bcs, [time])
+
+    def write_many(self, colliders, vel_bcs, times):
+        """Enqueue the copy of one table per time in ``times`` (at most count)."""
+        if len(times) > self.count:
+            raise ValueError(f"{len(times)} tables requested, {self.count} held")
         i = self._i
         self._i = (i + 1) % len(self._host)
         if self._events[i] is not None:
             self._events[i].synchronize()
-        C.memmove(self._host[i].data_ptr(), C.addressof(self.table), C.sizeof(self.table))
-        self.dev.copy_(self._host[i], non_blocking=True)
+        base = self._host[i].data_ptr()
+        for k, t in enumerate(times):
+            describe_boundaries(colliders, vel_bcs, t, self.table)
+            C.memmove(base + k * self.nbytes, C.addressof(self.table), self.nbytes)
+        n = len(times) * self.nbytes
+        self.dev[:n].copy_(self._host[i][:n], non_blocking=True)
         ev = torch.cuda.Event()
         ev.record(torch.cuda.current_stream(self.device))
         self._events[i] = ev

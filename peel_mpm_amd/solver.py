@@ -30,17 +30,24 @@ import math
 import torch
 
 from . import _lib
-from .boundaries import HalfspaceCollider, SphereCollider, VelocityBC
+from .boundaries import DeviceBoundaryTable, HalfspaceCollider, SphereCollider, VelocityBC
 
 _STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
 _FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "residual", "A0", "n_interface")
+
+
+def _launch_shape():
+    c = _lib.LaunchConfig()
+    _lib.lib().pmpm_get_launch_config(C.byref(c))
+    return [getattr(c, k) for k, _ in _lib.LaunchConfig._fields_]
 
 
 class AnisoMPM:
     def __init__(self, n_grid: int = 96, grid_lim: float = 1.0, dt: float = 1e-4, gravity=(0.0, -9.8, 0.0),
                  grid_damp: float = 1.0, f_clamp=None, damage_every: int = 1, device: str = "cuda:0",
                  dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = True,
-                 damage_solver: str = "explicit", pcg_tol: float = 1e-6, pcg_max_iter: int = 50):
+                 damage_solver: str = "explicit", pcg_tol: float = 1e-6, pcg_max_iter: int = 50,
+                 use_graph: bool = False):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the native AnisoMPM step runs on a ROCm GPU (device='cuda:N')")
@@ -76,6 +83,8 @@ class AnisoMPM:
         self._bc_key = None
         self._bc_id = None
         self._p, self._p_key = None, None
+        self.use_graph = bool(use_graph)
+        self._graph = None
         self._pin = _lib.ParticlesIn()
         self._out_n = -1
         _lib.lib()
@@ -342,8 +351,136 @@ class AnisoMPM:
         gv[lin, :3] = vs
 
     def run_frame(self, n_sub: int, time0: float = 0.0):
+        """substep(time0 + s dt) for s < n_sub (anisompm.py:805-807).  With
+        use_graph=True and a frame the device can run on its own (every
+        particle_bc a VelocityBC, every collider a device collider, the
+        explicit damage update), the n_sub substeps are captured once as one
+        HIP graph and replayed: see _run_frame_graph."""
+        if n_sub > 0 and self.use_graph and self._graph_eligible():
+            self._run_frame_graph(int(n_sub), float(time0))
+            return
         for s in range(n_sub):
             self.substep(time0 + s * self.dt)
+
+    # ------------------------------------------------------------------ captured frames
+    def _graph_eligible(self):
+        return (self.damage_solver == "explicit" and len(self.colliders) <= _lib.MAX_COLLIDERS
+                and len(self.particle_bc) <= _lib.MAX_VEL_BC
+                and all(isinstance(b, VelocityBC) for b in self.particle_bc)
+                and all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders))
+
+    def _run_frame_graph(self, n_sub, time0):
+        """One frame of n_sub substeps replayed from a captured HIP graph.
+
+        The state lives in two persistent ping-pong buffers (the attributes
+        are rebound to the current one after the frame; a caller's in-place
+        edits of them, or new tensors bound to them, are taken up at the
+        next frame), the diagnostics in persistent buffers written in place.
+        Before each replay the host writes, outside the graph, one
+        boundary table per substep (colliders and BC velocities at that
+        substep's time) and the per-substep damage gate, allow.any() on the
+        substeps damage_every selects, formed on the device; so moving
+        colliders, time-dependent velocities, edits of allow and
+        damage_every > 1 hold under replay.  A frame is captured again when a
+        parameter tensor is replaced or the configuration changes."""
+        dev, dt = self.device, self.dtype
+        bc_id, _ = self._velocity_bcs(time0)
+        for k in _STATE + _FLOAT_PARAMS:
+            self._tensor(k)
+        allow = self._tensor("allow", torch.bool)
+        imask = self._tensor("interface_mask", torch.bool)
+        ws = self._workspace()
+        G = self._graph
+        if G is None or G["n"] != self.n or G["count"] < n_sub:
+            n = self.n
+            bufs = [{k: torch.empty_like(getattr(self, k)) for k in _STATE} for _ in range(2)]
+            diag = {"tau_last": torch.zeros(n, 3, 3, device=dev, dtype=dt), "J_last": torch.ones(n, device=dev, dtype=dt),
+                    "phi": torch.zeros(n, device=dev, dtype=dt), "phi_shear": torch.zeros(n, device=dev, dtype=dt)}
+            for k in diag:
+                if hasattr(self, k) and getattr(self, k).shape == diag[k].shape:
+                    diag[k].copy_(getattr(self, k))
+            count = max(n_sub, G["count"] if G else 0)
+            G = self._graph = {"n": n, "count": count, "bufs": bufs, "diag": diag, "cur": 0, "graphs": {},
+                               "tab": DeviceBoundaryTable(dev, count),
+                               "gate": torch.zeros(count, dtype=torch.uint8, device=dev), "pattern": {}}
+        bufs, cur = G["bufs"], G["cur"]
+        for k in _STATE:                     # the caller's state into the current buffer
+            t = getattr(self, k)
+            if t.data_ptr() != bufs[cur][k].data_ptr():
+                bufs[cur][k].copy_(t)
+        for k in ("phi", "phi_shear", "tau_last", "J_last"):
+            t = getattr(self, k, None)
+            if t is not None and t.data_ptr() != G["diag"][k].data_ptr() and t.shape == G["diag"][k].shape:
+                G["diag"][k].copy_(t)
+        G["tab"].write_many(self.colliders, self.particle_bc, [time0 + s * self.dt for s in range(n_sub)])
+        pk = (n_sub, self._step % self.damage_every)
+        if pk not in G["pattern"]:
+            G["pattern"][pk] = torch.tensor([(self._step + s) % self.damage_every == 0 for s in range(n_sub)],
+                                            dtype=torch.uint8).to(dev)
+        torch.mul(G["pattern"][pk], allow.any().to(torch.uint8), out=G["gate"][:n_sub])
+        key = (n_sub, cur, self._config_key(), len(self.colliders), len(self.particle_bc),
+               bc_id.data_ptr() if bc_id is not None else 0, allow.data_ptr(), imask.data_ptr(), ws.data_ptr(),
+               tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS), tuple(_launch_shape()))
+        g = G["graphs"].get(key)
+        if g is None:
+            G["graphs"] = {k2: v for k2, v in G["graphs"].items() if k2[2:] == key[2:]}
+            launches = self._frame_launches(n_sub, cur, bc_id, allow, imask, ws)
+            s = torch.cuda.Stream(device=dev)
+            s.wait_stream(torch.cuda.current_stream(dev))
+            with torch.cuda.stream(s):
+                launches[0]()                # warm-up: one substep, src -> dst only
+            torch.cuda.current_stream(dev).wait_stream(s)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for f in launches:
+                    f()
+            G["graphs"][key] = g
+        g.replay()
+        cur = (cur + n_sub) % 2
+        G["cur"] = cur
+        for k in _STATE:
+            setattr(self, k, bufs[cur][k])
+        for k, t in G["diag"].items():
+            setattr(self, k, t)
+        self._step += n_sub
+
+    def _frame_launches(self, n_sub, cur, bc_id, allow, imask, ws):
+        """One closure per substep of a frame, each enqueueing pmpm_substep on
+        the current stream with its ping-pong buffers, boundary table and
+        damage gate entry."""
+        L = _lib.lib()
+        G = self._graph
+        base = self._step_params()
+        out = []
+        for s in range(n_sub):
+            src, dst = G["bufs"][(cur + s) % 2], G["bufs"][(cur + s + 1) % 2]
+            p = _lib.StepParams()
+            C.memmove(C.addressof(p), C.addressof(base), C.sizeof(p))
+            p.do_damage = 1
+            p.scene_damage = G["gate"].data_ptr() + s
+            p.boundary_table = G["tab"].entry(s)
+            p.n_colliders = len(self.colliders)
+            p.n_vel_bc = len(self.particle_bc)
+            p.export_active = 0
+            pin = _lib.ParticlesIn()
+            pin.n = self.n
+            for k in _STATE:
+                setattr(pin, k, src[k].data_ptr())
+            for k in _FLOAT_PARAMS:
+                setattr(pin, k, getattr(self, k).data_ptr())
+            pin.allow, pin.interface_mask = allow.data_ptr(), imask.data_ptr()
+            pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
+            pout = _lib.ParticlesOut()
+            for k in _STATE:
+                setattr(pout, k, dst[k].data_ptr())
+            for k, t in G["diag"].items():
+                setattr(pout, k, t.data_ptr())
+
+            def launch(p=p, pin=pin, pout=pout):
+                _lib.check(L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), ws.data_ptr(), ws.numel(),
+                                          _lib.stream_ptr(self.device)), "pmpm_substep")
+            out.append(launch)
+        return out
 
     # ------------------------------------------------------------------ diagnostics
     def last_step_stats(self):
