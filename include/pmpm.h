@@ -248,9 +248,13 @@ size_t pmpm_adjoint_workspace_bytes(int64_t max_n, int32_t n_grid, int32_t dtype
  * are read, d, D_hist, phi, phi_shear, tau_last, J_last are rewritten by the
  * recompute (with the stable sort, to the same values) unless
  * p->keep_for_adjoint reuses the forward call's workspace.  p must have
- * do_damage as in the forward call.  Supported: no f_clamp, no colliders,
- * one scene, no interface particles (the directional split is not
- * differentiated).  Velocity-BC particles (bc_id) get v_bar = 0. */
+ * do_damage as in the forward call.  Supported: one scene, the explicit
+ * damage update, f_clamp (through the strain limit's spectral derivative),
+ * device colliders (sticky, or slip with friction; torch's subgradient
+ * conventions at the clamp kinks); not interface particles (the directional
+ * split is not differentiated).  Velocity-BC particles (bc_id) get
+ * v_bar = 0.  With keep_for_adjoint the forward call keeps per node m, sum w
+ * and mv/m in ws for it. */
 int pmpm_substep_adjoint(const pmpm_step_params *p, const pmpm_particles_in *in,
                          const pmpm_particles_out *fwd_out, const pmpm_adjoint_in *gin,
                          const pmpm_adjoint_out *gout, void *ws, size_t ws_bytes, void *aws,

@@ -18,8 +18,10 @@ quantities instead: the polar rotation through (s_i + s_j) denominators and
 the tensile stress through Daleckii-Krein divided differences, with the
 f'(lambda) limit at equal eigenvalues, so it stays finite at F = I.
 
-Supported configurations: damage_every = 1, no f_clamp, no colliders, no
-interface particles; VelocityBC particle boundary conditions.
+Supported configurations: damage_every = 1, f_clamp, device colliders
+(sphere_collider / halfspace_collider, sticky or slip with friction, moving
+or not), VelocityBC particle boundary conditions; not interface particles
+(the directional stress split).
 """
 from __future__ import annotations
 
@@ -28,7 +30,7 @@ import ctypes as C
 import torch
 
 from . import _lib
-from .boundaries import VelocityBC
+from .boundaries import HalfspaceCollider, SphereCollider, VelocityBC
 from .solver import AnisoMPM
 
 STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
@@ -70,8 +72,11 @@ class StepRunner:
 
     def __init__(self, template: AnisoMPM, time0: float | None = None):
         sim = template
-        if sim.f_clamp is not None or sim.colliders or sim.damage_every != 1:
-            raise NotImplementedError("the adjoint supports damage_every=1, no f_clamp and no colliders")
+        if sim.damage_every != 1:
+            raise NotImplementedError("the adjoint supports damage_every=1")
+        if len(sim.colliders) > _lib.MAX_COLLIDERS or \
+                not all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in sim.colliders):
+            raise NotImplementedError("the adjoint takes device colliders (sphere_collider / halfspace_collider) only")
         if getattr(sim, "damage_solver", "explicit") != "explicit":
             raise NotImplementedError("the adjoint differentiates the explicit damage update only")
         if bool(sim.interface_mask.any()):
@@ -109,19 +114,21 @@ class StepRunner:
 
     def _step_params(self, k):
         sim = self.sim
-        bc_id, bc_vals = sim._velocity_bcs(self.time0 + k * sim.dt)
+        time = self.time0 + k * sim.dt
+        bc_id, bc_vals = sim._velocity_bcs(time)
         p = _lib.StepParams()
-        p.dtype, p.n_grid = sim._code, sim.n_grid
-        p.grid_lim, p.dx, p.inv_dx, p.dt = sim.grid_lim, sim.dx, sim.inv_dx, sim.dt
-        p.gravity[:] = sim._gravity_host
-        p.grid_damp = sim.grid_damp
+        C.memmove(C.addressof(p), C.addressof(sim._step_params()), C.sizeof(p))
         p.do_damage = int(self.do_damage)
         p.dt_damage = sim.dt
+        p.n_colliders = len(sim.colliders)
+        for j, c in enumerate(sim.colliders):
+            c.describe(time, p.colliders[j])
+        p.export_active = 0
         p.n_vel_bc = len(bc_vals)
         for j, v in enumerate(bc_vals):
             p.vel_bc[j][:] = v
         p.stable_sort = 1
-        p.exact_sigma_eig = int(sim.exact_sigma_eig)
+        p.keep_for_adjoint = 0
         return p, bc_id
 
     def _pin(self, s, bc_id):

@@ -58,6 +58,7 @@ constexpr int GRID_WAVES = 8192;       // waves of the grid kernel
 constexpr int GRID_WG_COUNT = GRID_WAVES / 4;   // its 256-thread work-groups
 constexpr int G2P_PREFETCH = 2;        // particles per G2P thread fetched before the node staging
 constexpr int P2G_REC = 16;            // values per P2G record: x(3), affine(9), mass*v(3), mass
+constexpr int GMW = 5;                 // values per node kept for the adjoint: m, sum w, mv/m (3)
 // PCG damage scalars (doubles in the pcgr section after the partial dots)
 enum { PCG_RZ = 0, PCG_BB, PCG_ALPHA, PCG_BETA, PCG_REL, PCG_ITER, PCG_DONE, PCG_NSCAL = 8 };
 
@@ -167,7 +168,7 @@ struct Layout {
         gv = o;     o = al(o + (size_t)G * 4 * ts);
         active = o; o = al(o + (size_t)G);
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
-        gmw = o;    o = al(o + (size_t)G * 2 * ts);                // per node (m, sum w) kept for the adjoint
+        gmw = o;    o = al(o + (size_t)G * GMW * ts);              // per node (m, sum w, mv/m) kept for the adjoint
         // ---- sections sized by n
         arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
         key = o;    o = al(o + np * 4);
@@ -511,7 +512,7 @@ template <typename T> struct StepArgs {
     T *ox, *ov, *oF, *oC, *olap, *od, *oD, *ophi, *ophs, *otau, *oJ;
     // workspace
     T *arena, *gv;
-    T* gmw;                     // optional: per node (m, sum w) of the grid pass
+    T* gmw;                     // optional: per node (m, sum w, mv/m) of the grid pass
     T* pdiag;                   // PCG damage: per node (Jacobi diagonal, lumped mass)
     double *ppart, *pscal;      // PCG damage: per-work-group partial dots, scalars
     uint8_t* active;
@@ -658,6 +659,38 @@ __device__ __forceinline__ M3<T> degrade(T res, const M3<T>& F, T d, T J, const 
         return tau;
 }
 
+// The strain limit (anisompm.py:622-636): F <- F V diag(clamp(S)/S) V^T from
+// the eigenvectors V of F^T F and the singular values S, then S <- clamp(S).
+template <typename T> __device__ __forceinline__ void fclamp_apply(const StepArgs<T>& A, const M3<T>& V, M3<T>& F, T S[3]) {
+    T sc[3], r[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        sc[i] = minp(maxp(S[i], A.fc_lo), A.fc_hi);
+        r[i] = sc[i] / maxp(S[i], T(1e-12));
+    }
+    M3<T> VD;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) VD.m[i][j] = V.m[i][j] * r[j];
+    F = mul(F, mul_bt(VD, V));
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S[i] = sc[i];
+}
+
+// The strain-limited F of a particle as the constitutive update forms it.
+template <typename T> __device__ __forceinline__ M3<T> fclamped(const StepArgs<T>& A, const M3<T>& F) {
+    if (!A.use_fclamp) return F;
+    M3<T> Cm = mul_at(F, F), V;
+    T lc[3], S[3];
+    sym_eig3<T, true>(Cm, lc, V);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) S[i] = sqrt(maxp(lc[i], T(1e-12)));
+    M3<T> Fc = F;
+    fclamp_apply(A, V, Fc, S);
+    return Fc;
+}
+
 // Stress, damage and degradation of one particle (anisompm.py:614-706).
 // Returns the degraded Kirchhoff stress; F is replaced by its clamped value
 // when f_clamp is on; d becomes the post-update damage.  DRIVE: stop after
@@ -681,22 +714,7 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
         for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] * inv;
     }
     M3<T> R = mul_bt(Wh, V);
-    if (A.use_fclamp) {
-        T sc[3], r[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            sc[i] = minp(maxp(S[i], A.fc_lo), A.fc_hi);
-            r[i] = sc[i] / maxp(S[i], T(1e-12));
-        }
-        M3<T> VD;
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) VD.m[i][j] = V.m[i][j] * r[j];
-        F = mul(F, mul_bt(VD, V));
-#pragma unroll
-        for (int i = 0; i < 3; ++i) S[i] = sc[i];
-    }
+    if (A.use_fclamp) fclamp_apply(A, V, F, S);
     J = maxp(S[0] * S[1] * S[2], T(1e-6));
     const T mu = q.mu, lam = q.lam;
     M3<T> FmR;
@@ -1161,6 +1179,27 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
     }
 }
 
+// One collider at a node position: its velocity (the forward difference of
+// its placement, anisompm.py:837-839, :861-863), the outward normal there and
+// whether the node is inside it (anisompm.py:828-867).
+template <typename T>
+__device__ __forceinline__ bool collider_setup(const pmpm_collider& cl, const T pos[3], T vc[3], T n[3]) {
+    const T p0[3] = {T(cl.p0[0]), T(cl.p0[1]), T(cl.p0[2])};
+    const T p1[3] = {T(cl.p1[0]), T(cl.p1[1]), T(cl.p1[2])};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vc[a] = (p1[a] - p0[a]) / T(1e-4);
+    if (cl.kind == PMPM_COLLIDER_SPHERE) {
+        const T r[3] = {pos[0] - p0[0], pos[1] - p0[1], pos[2] - p0[2]};
+        const T dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        const T den = dist + T(1e-9);
+        n[0] = r[0] / den; n[1] = r[1] / den; n[2] = r[2] / den;
+        return dist < T(cl.radius);
+    }
+    n[0] = T(cl.normal[0]); n[1] = T(cl.normal[1]); n[2] = T(cl.normal[2]);
+    const T sd = (pos[0] - p0[0]) * n[0] + (pos[1] - p0[1]) * n[1] + (pos[2] - p0[2]) * n[2];
+    return sd < T(0);
+}
+
 // Grid velocity update and colliders per marked 4x4x4 node block
 // (anisompm.py:726-746), cubic normalisation (:777-779).  The nodes of block
 // (bx,by,bz) lie in the arenas [4b-1, 4b+5] of buckets b = bx-1..bx+1 per
@@ -1203,24 +1242,8 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
                 if (A.damp != T(1)) { v[0] = v[0] * A.damp; v[1] = v[1] * A.damp; v[2] = v[2] * A.damp; }
                 const T pos[3] = {T(gx) * A.dx, T(gy) * A.dx, T(gz) * A.dx};
                 for (int k = 0; k < A.n_col; ++k) {
-                    const pmpm_collider& cl = s_col[k];
-                    const T p0[3] = {T(cl.p0[0]), T(cl.p0[1]), T(cl.p0[2])};
-                    const T p1[3] = {T(cl.p1[0]), T(cl.p1[1]), T(cl.p1[2])};
-                    const T vc[3] = {(p1[0] - p0[0]) / T(1e-4), (p1[1] - p0[1]) / T(1e-4), (p1[2] - p0[2]) / T(1e-4)};
-                    T n[3];
-                    bool inside;
-                    if (cl.kind == PMPM_COLLIDER_SPHERE) {
-                        const T r[3] = {pos[0] - p0[0], pos[1] - p0[1], pos[2] - p0[2]};
-                        const T dist = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-                        inside = dist < T(cl.radius);
-                        const T den = dist + T(1e-9);
-                        n[0] = r[0] / den; n[1] = r[1] / den; n[2] = r[2] / den;
-                    } else {
-                        n[0] = T(cl.normal[0]); n[1] = T(cl.normal[1]); n[2] = T(cl.normal[2]);
-                        const T sd = (pos[0] - p0[0]) * n[0] + (pos[1] - p0[1]) * n[1] + (pos[2] - p0[2]) * n[2];
-                        inside = sd < T(0);
-                    }
-                    if (inside) collider_resolve(v, vc, n, cl.mode, T(cl.friction));
+                    T vc[3], n[3];
+                    if (collider_setup<T>(s_col[k], pos, vc, n)) collider_resolve(v, vc, n, s_col[k].mode, T(s_col[k].friction));
                 }
             }
             const bool cact = wsum > T(1e-12);
@@ -1228,7 +1251,13 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
             g[0] = v[0]; g[1] = v[1]; g[2] = v[2];
             g[3] = cact ? dsum / wsum : T(0);
             if (A.export_active) A.active[node] = act ? 1 : 0;
-            if (A.gmw) { A.gmw[2 * node] = m; A.gmw[2 * node + 1] = wsum; }
+            if (A.gmw) {
+                T* gw = A.gmw + GMW * node;
+                gw[0] = m; gw[1] = wsum;
+                gw[2] = act ? mvx * (T(1) / m) : T(0);
+                gw[3] = act ? mvy * (T(1) / m) : T(0);
+                gw[4] = act ? mvz * (T(1) / m) : T(0);
+            }
             nq += act;
             nc += cact;
         }
@@ -2649,9 +2678,8 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
     if (aws_bytes < pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype))
         return fail(PMPM_ESIZE, "%sadjoint workspace too small, need %lld bytes", "",
                     (long long)pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype));
-    if (p->use_fclamp || p->n_colliders > 0 || p->n_scenes > 1 || p->damage_solver != PMPM_DAMAGE_EXPLICIT)
-        return fail(PMPM_EARG, "%sthe adjoint supports no f_clamp, no colliders, one scene and the explicit damage update",
-                    "");
+    if (p->n_scenes > 1 || p->damage_solver != PMPM_DAMAGE_EXPLICIT || p->export_active)
+        return fail(PMPM_EARG, "%sthe adjoint supports one scene, device colliders and the explicit damage update", "");
     if (in->n > 0) {
         const void* need[] = {gin->x, gin->v, gin->F, gin->C, gin->d, gin->lap, gin->D_hist, gout->x, gout->v, gout->F,
                               gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, gout->sigma_c, fwd_out->d,

@@ -17,8 +17,9 @@
 //                  (the affine matrix and d_new come from the recompute's
 //                  P2G records)
 //   k_adj_const    per particle: constitutive VJP and the F update
-// Supported: no f_clamp, no colliders, no directional (interface) stress,
-// damage_every = 1 -- the configuration of the differentiable benchmark.
+// Supported: the strain limit (f_clamp), device colliders (sticky and
+// slip with friction), velocity BCs, damage_every = 1; not the directional
+// (interface) stress.
 // Subgradients follow torch's conventions so the result matches autograd of
 // the reference step: relu'(0) = 0, clamp passes the gradient on the closed
 // interval, maximum splits it 1/2 : 1/2 at ties.
@@ -70,13 +71,62 @@ template <typename T> __device__ __forceinline__ void max_grad(T a, T b, T g, T&
     else { ga = T(0.5) * g; gb = T(0.5) * g; }
 }
 
+// Adjoint of collider_resolve (anisompm.py:811-825) at input velocity v:
+// g holds the adjoint of the resolved velocity on entry and that of v on
+// exit (the collider velocity and normal do not depend on the state).
+// torch's conventions: clamp(max=0) of the normal part and clamp(min=0) of
+// the friction factor pass the gradient on their closed intervals; the norm
+// of a zero tangential part passes none.
+template <typename T>
+__device__ __forceinline__ void collider_resolve_vjp(const T v[3], const T vc[3], const T n[3], int mode, T friction,
+                                                     T g[3]) {
+    if (mode == 0) {
+        g[0] = g[1] = g[2] = T(0);      // sticky: the result is the collider velocity
+        return;
+    }
+    const T vr[3] = {v[0] - vc[0], v[1] - vc[1], v[2] - vc[2]};
+    const T vn = vr[0] * n[0] + vr[1] * n[1] + vr[2] * n[2];
+    const T inward = minp(vn, T(0));
+    const T vt[3] = {vr[0] - vn * n[0], vr[1] - vn * n[1], vr[2] - vn * n[2]};
+    // v' = vc + vt' + (vn - inward) n
+    const T keep_bar = g[0] * n[0] + g[1] * n[1] + g[2] * n[2];
+    T vn_bar = keep_bar, inward_bar = -keep_bar;
+    T vt_bar[3];
+    if (friction > T(0)) {
+        const T mag = sqrt(vt[0] * vt[0] + vt[1] * vt[1] + vt[2] * vt[2]);
+        const T den = mag + T(1e-9);
+        const T raw = T(1) + friction * inward / den;
+        const T sc = maxp(raw, T(0));
+        const T sc_bar = g[0] * vt[0] + g[1] * vt[1] + g[2] * vt[2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vt_bar[a] = g[a] * sc;
+        const T raw_bar = raw >= T(0) ? sc_bar : T(0);
+        inward_bar += raw_bar * friction / den;
+        const T mag_bar = -raw_bar * friction * inward / (den * den);
+        if (mag > T(0)) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) vt_bar[a] += mag_bar * vt[a] / mag;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) vt_bar[a] = g[a];
+    }
+    if (vn <= T(0)) vn_bar += inward_bar;
+    vn_bar -= vt_bar[0] * n[0] + vt_bar[1] * n[1] + vt_bar[2] * n[2];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) g[a] = vt_bar[a] + vn_bar * n[a];
+}
+
 // Constitutive VJP of one particle (no f_clamp, no interface).  Recomputes
 // the forward of anisompm.py:614-691 in registers, then back-propagates
 // tau_bar (degraded Kirchhoff stress), dn_bar (post-update damage) and
 // Dn_bar (post-update history drive) to F, d, lap, D_hist, mu, lam, A0.
+// With the strain limit, F is the step's input and Fk the limited F the
+// stress and the F update use; Fk_bar_in is the adjoint reaching Fk from the
+// F update, and F_bar comes out with respect to the input F.
 template <typename T>
 __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>& F, bool dmg_on,
-                                 const M3<T>& tau_bar, T dn_bar, T Dn_bar,
+                                 const M3<T>& tau_bar, T dn_bar, T Dn_bar, const M3<T>& Fk_bar_in,
                                  M3<T>& F_bar, T& d_bar, T& lap_bar, T& Dh_bar, T& mu_bar, T& lam_bar,
                                  M3<T>& A0_bar, T& sc_bar) {
     // ---- forward
@@ -95,15 +145,19 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
         for (int i = 0; i < 3; ++i) Wh.m[i][j] = W.m[i][j] * inv;
     }
     const M3<T> R = mul_bt(Wh, V);
-    const T J0 = S[0] * S[1] * S[2];
+    // the strain limit (anisompm.py:622-636): Fk = F V diag(clamp(S)/S) V^T, Sk = clamp(S)
+    M3<T> Fk = F;
+    T Sk[3] = {S[0], S[1], S[2]};
+    if (A.use_fclamp) fclamp_apply(A, V, Fk, Sk);
+    const T J0 = Sk[0] * Sk[1] * Sk[2];
     const T J = maxp(J0, T(1e-6));
     const T mu = A.mu[p], lam = A.lam[p];
     M3<T> FmR;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) FmR.m[i][j] = F.m[i][j] - R.m[i][j];
-    const M3<T> P0 = mul_bt(FmR, F);       // (F - R) F^T
+        for (int j = 0; j < 3; ++j) FmR.m[i][j] = Fk.m[i][j] - R.m[i][j];
+    const M3<T> P0 = mul_bt(FmR, Fk);      // (Fk - R) Fk^T
     M3<T> tdev;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -259,32 +313,41 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
     // tvol = lam (J - 1) J
     lam_bar = tvol_bar * (J - T(1)) * J;
     J_bar += tvol_bar * lam * (T(2) * J - T(1));
-    // tdev = 2 mu (F - R) F^T
+    // tdev = 2 mu (Fk - R) Fk^T
     mu_bar = T(2) * ddot3(tdev_bar, P0);
-    const M3<T> a1 = mul(tdev_bar, F), a2 = mul_at(tdev_bar, FmR);
+    const M3<T> a1 = mul(tdev_bar, Fk), a2 = mul_at(tdev_bar, FmR);
+    M3<T> Fk_bar;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            F_bar.m[i][j] = T(2) * mu * (a1.m[i][j] + a2.m[i][j]);
+            Fk_bar.m[i][j] = T(2) * mu * (a1.m[i][j] + a2.m[i][j]) + Fk_bar_in.m[i][j];
             R_bar.m[i][j] -= T(2) * mu * a1.m[i][j];
+            F_bar.m[i][j] = T(0);
         }
-    // J = max(S0 S1 S2, 1e-6), S_i = sqrt(lambda_i(F^T F)):  dJ/dF = J F V diag(1/lambda) V^T
+    // J = max(Sk0 Sk1 Sk2, 1e-6), Sk_i = clamp(S_i) (or S_i), S_i = sqrt(max(lambda_i(F^T F), 1e-12)):
+    // lambda_bar_i = J_bar (prod of the other Sk) [S_i in the clamp range] / (2 S_i), F_bar += 2 F V diag(lambda_bar) V^T
     const T J0_bar = J0 >= T(1e-6) ? J_bar : T(0);
     if (J0_bar != T(0)) {
+        T lb[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const T others = Sk[(i + 1) % 3] * Sk[(i + 2) % 3];
+            const bool pass = !A.use_fclamp || (S[i] >= A.fc_lo && S[i] <= A.fc_hi);
+            lb[i] = (pass && lc[i] >= T(1e-12)) ? J0_bar * others / (T(2) * S[i]) : T(0);
+        }
         M3<T> VD;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                VD.m[i][j] = V.m[i][j] * (lc[j] >= T(1e-12) ? J0 / lc[j] : T(0));
+            for (int j = 0; j < 3; ++j) VD.m[i][j] = V.m[i][j] * lb[j];
         const M3<T> dJ = mul(F, mul_bt(VD, V));
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) F_bar.m[i][j] += J0_bar * dJ.m[i][j];
+            for (int j = 0; j < 3; ++j) F_bar.m[i][j] += T(2) * dJ.m[i][j];
     }
-    // polar rotation: dR = R Omega, (V^T Omega V)_ij = (X_ij - X_ji)/(s_i + s_j),
+    // polar rotation (of the input F): dR = R Omega, (V^T Omega V)_ij = (X_ij - X_ji)/(s_i + s_j),
     // X = V^T R^T dF V  =>  F_bar += R V H V^T, H_ij = (G_ij - G_ji)/(s_i + s_j), G = V^T R^T R_bar V
     const M3<T> Gp = mul(mul_at(V, mul_at(R, R_bar)), V);
     M3<T> H;
@@ -300,10 +363,54 @@ __device__ void constitutive_vjp(const StepArgs<T>& A, long long p, const M3<T>&
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) F_bar.m[i][j] += fr.m[i][j];
+    if (!A.use_fclamp) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) F_bar.m[i][j] += Fk_bar.m[i][j];
+        return;
+    }
+    // Fk = F Hk, Hk = g(F^T F) = V diag(g(lambda)) V^T, g(lambda) = clamp(S)/max(S, 1e-12):
+    // F_bar += Fk_bar Hk + 2 F C_bar, C_bar = V (Gamma o V^T sym(F^T Fk_bar) V) V^T with the
+    // divided differences Gamma_ij = (g_i - g_j)/(lambda_i - lambda_j), g'(lambda_i) at ties
+    T gv[3], gd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const T Sm = maxp(S[i], T(1e-12));
+        gv[i] = Sk[i] / Sm;
+        const bool pass = S[i] >= A.fc_lo && S[i] <= A.fc_hi;
+        const T drdS = (pass ? T(1) : T(0)) / Sm - (S[i] >= T(1e-12) ? Sk[i] / (Sm * Sm) : T(0));
+        gd[i] = lc[i] >= T(1e-12) ? drdS / (T(2) * S[i]) : T(0);
+    }
+    M3<T> VG;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) VG.m[i][j] = V.m[i][j] * gv[j];
+    const M3<T> Hk = mul_bt(VG, V);
+    const M3<T> t3 = mul(Fk_bar, Hk);
+    const M3<T> Hb = mul_at(F, Fk_bar);
+    M3<T> Hs;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) Hs.m[i][j] = T(0.5) * (Hb.m[i][j] + Hb.m[j][i]);
+    M3<T> K = mul(mul_at(V, Hs), V);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) K.m[i][j] *= lc[i] != lc[j] ? (gv[i] - gv[j]) / (lc[i] - lc[j]) : gd[i];
+    const M3<T> Cb = mul_bt(mul(V, K), V);
+    const M3<T> t4 = mul(F, Cb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F_bar.m[i][j] += t3.m[i][j] + T(2) * t4.m[i][j];
 }
 
 // v'_bar and C'_bar seen by the transfer: the advection x' = clamp(x + dt v')
-// and F' = (I + dt C') F routes part of x'_bar and F'_bar into them.
+// and F' = (I + dt C') F routes part of x'_bar and F'_bar into them (F: the
+// strain-limited F when f_clamp is on).
 template <typename T>
 __device__ __forceinline__ void transfer_bars(const StepArgs<T>& A, const AdjArgs<T>& B, long long p, const T xp[3],
                                               const M3<T>& F, T vbar[3], M3<T>& Cbar, bool xin[3]) {
@@ -346,7 +453,7 @@ __global__ void __launch_bounds__(P2G_WG) k_adj_scatter(StepArgs<T> A, AdjArgs<T
             if (wv >= NW || r >= cnt) continue;
             const long long p = A.perm[c0 + r];
             const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
-            const M3<T> F = load_m3(A.F, p);
+            const M3<T> F = fclamped(A, load_m3(A.F, p));
             T vb[3];
             M3<T> Cb;
             bool xin[3];
@@ -413,6 +520,12 @@ __global__ void __launch_bounds__(P2G_WG) k_adj_scatter(StepArgs<T> A, AdjArgs<T
 //   gd = S/W (W > 1e-12)                 ->  S_bar = gd_bar / W, W_bar = -gd_bar gd / W
 template <typename T>
 __global__ void __launch_bounds__(WG) k_adj_grid(StepArgs<T> A, AdjArgs<T> B) {
+    __shared__ pmpm_collider s_col[PMPM_MAX_COLLIDERS];
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < PMPM_MAX_COLLIDERS; ++k) s_col[k] = A.btab ? A.btab->colliders[k] : A.col[k];
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = (int)((blockIdx.x * (unsigned)WG + threadIdx.x) >> 6);
     const int nwaves = gridDim.x * (WG / 64);
@@ -452,17 +565,41 @@ __global__ void __launch_bounds__(WG) k_adj_grid(StepArgs<T> A, AdjArgs<T> B) {
                     }
         }
         const long long node = scene * A.Gs + ((long long)g[0] * ng + g[1]) * ng + g[2];
-        const T m = B.gmw[2 * node], Wsum = B.gmw[2 * node + 1];
+        const T* gw = B.gmw + GMW * node;
+        const T m = gw[0], Wsum = gw[1];
         const T* gv = A.gv + 4 * node;
         T out[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
         if (m > T(1e-12)) {
+            const T q[3] = {gw[2], gw[3], gw[4]};          // mv / m
+            T vb[3] = {acc[0], acc[1], acc[2]};             // adjoint of the node's final velocity
+            if (A.n_col > 0) {
+                // back through the colliders in reverse order, each from the
+                // velocity that entered it (the grid pass: gravity, damping,
+                // then the colliders in list order, anisompm.py:730-746)
+                T v0[3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    v0[a] = q[a] + A.dtg[a];
+                    if (A.damp != T(1)) v0[a] = v0[a] * A.damp;
+                }
+                const T pos[3] = {T(g[0]) * A.dx, T(g[1]) * A.dx, T(g[2]) * A.dx};
+                for (int k = A.n_col - 1; k >= 0; --k) {
+                    T vk[3] = {v0[0], v0[1], v0[2]};
+                    for (int j = 0; j < k; ++j) {
+                        T vc[3], nj[3];
+                        if (collider_setup<T>(s_col[j], pos, vc, nj)) collider_resolve(vk, vc, nj, s_col[j].mode, T(s_col[j].friction));
+                    }
+                    T vc[3], nk[3];
+                    if (collider_setup<T>(s_col[k], pos, vc, nk))
+                        collider_resolve_vjp(vk, vc, nk, s_col[k].mode, T(s_col[k].friction), vb);
+                }
+            }
             T vbar_dot = T(0);
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
-                const T vb = A.damp * acc[a];
-                out[a] = vb / m;
-                const T vpre = gv[a] / A.damp - A.dtg[a];      // mv / m
-                vbar_dot += vb * vpre;
+                const T vbd = A.damp * vb[a];
+                out[a] = vbd / m;
+                vbar_dot += vbd * q[a];
             }
             out[3] = -vbar_dot / m;
         }
@@ -487,7 +624,7 @@ __global__ void __launch_bounds__(WG) k_adj_gather(StepArgs<T> A, AdjArgs<T> B) 
     // scene of the particle's grid (single scene in the adjoint)
     const long long gbase = 0;
     const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
-    const M3<T> F = load_m3(A.F, p);
+    const M3<T> F = fclamped(A, load_m3(A.F, p));
     T vb[3];
     M3<T> Cb;
     bool xin[3];
@@ -650,11 +787,7 @@ __global__ void __launch_bounds__(WG) k_adj_const(StepArgs<T> A, AdjArgs<T> B) {
 #pragma unroll
         for (int b = 0; b < 3; ++b) tau_bar.m[a][b] = pb[3 * a + b];
     const T dnb_cubic = pb[9];
-    M3<T> Fb, A0b;
-    T d_b, lap_b, Dh_b, mu_b, lam_b, sc_b;
-    const bool dmg_on = B.dmg_on != 0;
-    constitutive_vjp(A, p, F, dmg_on, tau_bar, B.gd[p] + dnb_cubic, B.gD[p], Fb, d_b, lap_b, Dh_b, mu_b, lam_b, A0b, sc_b);
-    // F' = (I + dt C') F
+    // F' = (I + dt C') Fk: its adjoint reaches the limited F
     const M3<T> Fbn = load_m3(B.gF, p);
     const M3<T> Cn = load_m3(B.nC, p);
     M3<T> Ipc;
@@ -662,11 +795,12 @@ __global__ void __launch_bounds__(WG) k_adj_const(StepArgs<T> A, AdjArgs<T> B) {
     for (int a = 0; a < 3; ++a)
 #pragma unroll
         for (int b = 0; b < 3; ++b) Ipc.m[a][b] = (a == b ? T(1) : T(0)) + A.dt * Cn.m[a][b];
-    const M3<T> Fb2 = mul_at(Ipc, Fbn);
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int b = 0; b < 3; ++b) Fb.m[a][b] += Fb2.m[a][b];
+    const M3<T> Fk_bar = mul_at(Ipc, Fbn);
+    M3<T> Fb, A0b;
+    T d_b, lap_b, Dh_b, mu_b, lam_b, sc_b;
+    const bool dmg_on = B.dmg_on != 0;
+    constitutive_vjp(A, p, F, dmg_on, tau_bar, B.gd[p] + dnb_cubic, B.gD[p], Fk_bar, Fb, d_b, lap_b, Dh_b, mu_b, lam_b,
+                     A0b, sc_b);
     B.od[p] = d_b;
     B.olap[p] = lap_b;
     B.oD[p] = Dh_b;

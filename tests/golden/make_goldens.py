@@ -18,7 +18,9 @@ in the tests):
   grid damping, colliders, particle BC), in fp32 and fp64;
 * a short multi-step trajectory of a small peel slab;
 * autograd through a 3-step fp64 rollout (the reference has no gradient
-  code of its own; this pins the hand-written adjoint).
+  code of its own; this pins the hand-written adjoint), and through a second
+  one with the strain limit (f_clamp) and moving colliders (a slip sphere
+  with friction, a sticky half-space).
 
 Run:  python tests/golden/make_goldens.py
 """
@@ -359,14 +361,61 @@ def _gold_adjoint(ref):
     np.savez_compressed(os.path.join(HERE, "adjoint_rollout64.npz"), **rec)
 
 
+def gold_adjoint_fclamp_colliders(ref):
+    """As gold_adjoint, on a scene with f_clamp = (0.97, 1.03) (F = I +
+    0.08 N(0,1) puts most singular values outside it), a slip sphere with
+    friction 0.3 moving down at 0.2 and a static sticky half-space under the
+    block."""
+    saved = ref.sym_eig
+    ref.sym_eig = grad_safe_sym_eig
+    try:
+        sim, rng = random_scene(ref, 9, torch.float64, ng=24, n_side=(8, 6, 8), gravity=(0.0, -9.8, 0.0),
+                                f_clamp=(0.97, 1.03))
+        n = sim.n
+        sim.sigma_c = sim.sigma_c * torch.tensor(rng.uniform(0.5, 40.0, n))
+        sc = np.array([0.5, 0.61, 0.5]); sv = np.array([0.0, -0.2, 0.0]); sr = 0.06
+        sim.colliders.append(ref.sphere_collider(lambda t: (sc + sv * t).tolist(), sr, mode="slip", friction=0.3))
+        ho = np.array([0.0, 0.475, 0.0]); hn = [0.0, 1.0, 0.0]
+        sim.colliders.append(ref.halfspace_collider(lambda t: ho.tolist(), hn, mode="sticky"))
+        leaves = {}
+        for k in ("x", "v", "F", "C", "d", "lap", "D_hist", "mu", "lam", "A0", "sigma_c"):
+            t = getattr(sim, k).detach().clone().requires_grad_(True)
+            setattr(sim, k, t)
+            leaves[k] = t
+        rec = {"in_" + k: np_(v) for k, v in leaves.items()}
+        for k in ("vol0", "mass", "eta", "zeta", "l0", "residual", "allow", "interface_mask", "n_interface"):
+            rec["in_" + k] = np_(getattr(sim, k))
+        target = leaves["x"].detach() + torch.tensor(rng.normal(0, 1e-3, (n, 3)))
+        for s_ in range(3):
+            sim.substep(s_ * sim.dt)
+        loss = ((sim.x - target) ** 2).sum()
+        loss.backward()
+        rec["target"] = np_(target)
+        rec["loss"] = np.float64(loss.item())
+        rec["out_x"] = np_(sim.x)
+        for k, t in leaves.items():
+            rec["grad_" + k] = np_(t.grad)
+        rec["cfg"] = np.array([sim.n_grid, sim.grid_lim, sim.dt, 3], np.float64)
+        rec["fclamp"] = np.array([0.97, 1.03])
+        rec["sph_c"], rec["sph_v"], rec["sph_r"] = sc, sv, np.float64(sr)
+        rec["hs_o"], rec["hs_n"] = ho, np.array(hn, np.float64)
+        np.savez_compressed(os.path.join(HERE, "adjoint_fclamp_colliders64.npz"), **rec)
+    finally:
+        ref.sym_eig = saved
+
+
 def main():
     torch.set_num_threads(1)
     ref = load_reference()
+    if sys.argv[1:] == ["adjoint_fclamp_colliders"]:
+        gold_adjoint_fclamp_colliders(ref)
+        return
     gold_sym_eig(ref)
     gold_weights(ref)
     gold_steps(ref)
     gold_traj(ref)
     gold_adjoint(ref)
+    gold_adjoint_fclamp_colliders(ref)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -27,11 +27,11 @@ STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
 FP32_ADJ_TOL = 3e-4
 
 
-def sim_from_golden(dtype=torch.float64):
+def sim_from_golden(dtype=torch.float64, name="adjoint_rollout64.npz", **kw):
     from peel_mpm_amd import AnisoMPM
-    z = load_npz("adjoint_rollout64.npz")
+    z = load_npz(name)
     ng, lim, dt, steps = z["cfg"].tolist()
-    sim = AnisoMPM(n_grid=int(ng), grid_lim=lim, dt=dt, device=DEV, dtype=dtype)
+    sim = AnisoMPM(n_grid=int(ng), grid_lim=lim, dt=dt, device=DEV, dtype=dtype, **kw)
     for k in STATE + ("mu", "lam", "A0", "vol0", "mass", "sigma_c", "eta", "zeta", "l0", "residual", "allow",
                       "interface_mask", "n_interface"):
         t = torch.from_numpy(z["in_" + k].copy())
@@ -60,6 +60,53 @@ def test_adjoint_matches_reference_autograd():
     for k in ("mu", "lam", "A0", "sigma_c"):
         report[k] = rel_err(grads[k], torch.from_numpy(z["grad_" + k]))
     print({k: f"{v:.2e}" for k, v in report.items()})
+    for k, e in report.items():
+        assert e <= 1e-6, (k, e, report)
+
+
+def _golden_colliders(z):
+    from peel_mpm_amd import halfspace_collider, sphere_collider
+    sc, sv, sr = z["sph_c"], z["sph_v"], float(z["sph_r"])
+    ho, hn = z["hs_o"], z["hs_n"].tolist()
+    return [sphere_collider(lambda t: (sc + sv * t).tolist(), sr, mode="slip", friction=0.3),
+            halfspace_collider(lambda t: ho.tolist(), hn, mode="sticky")]
+
+
+def test_adjoint_with_fclamp_and_colliders_matches_reference_autograd():
+    """The strain limit (f_clamp 0.97..1.03, active on most particles) and
+    moving device colliders (a slip sphere with friction, a sticky
+    half-space) on a damaging 3-step fp64 rollout: every state and parameter
+    gradient within 1e-6 of autograd through the reference step (golden
+    vectors, the reference's sym_eig swapped for a gradient-safe twin)."""
+    from peel_mpm_amd.adjoint import StepRunner, state_vjp
+    name = "adjoint_fclamp_colliders64.npz"
+    z = load_npz(name)
+    sim, z, steps = sim_from_golden(name=name, f_clamp=tuple(z["fclamp"].tolist()))
+    sim.colliders.extend(_golden_colliders(z))
+    target = torch.from_numpy(z["target"]).to(DEV)
+
+    def forward(s_):
+        r = StepRunner(s_)
+        st = {k: getattr(s_, k) for k in STATE}
+        for k in range(steps):
+            st = r.step(st, k)
+        return st["x"]
+    xT = forward(sim)
+    loss = float(((xT - target) ** 2).sum())
+    assert abs(loss - float(z["loss"])) <= 1e-9 * abs(float(z["loss"])), (loss, float(z["loss"]))
+    # the case exercises both: without either the loss moves far beyond rounding
+    for drop in ("f_clamp", "colliders"):
+        other, _, _ = sim_from_golden(name=name, **({} if drop == "f_clamp" else {"f_clamp": tuple(z["fclamp"].tolist())}))
+        if drop == "f_clamp":
+            other.colliders.extend(_golden_colliders(z))
+        l2 = float(((forward(other) - target) ** 2).sum())
+        print(f"without {drop}: loss {l2:.9e} vs {loss:.9e}", flush=True)
+        assert abs(l2 - loss) > 1e-6 * loss
+    _, adj, grads = state_vjp(sim, steps, 2.0 * (xT - target))
+    torch.cuda.synchronize()
+    report = {k: rel_err(adj[k], torch.from_numpy(z["grad_" + k])) for k in STATE}
+    report.update({k: rel_err(grads[k], torch.from_numpy(z["grad_" + k])) for k in ("mu", "lam", "A0", "sigma_c")})
+    print("f_clamp + colliders adjoint vs reference autograd", {k: f"{v:.2e}" for k, v in report.items()}, flush=True)
     for k, e in report.items():
         assert e <= 1e-6, (k, e, report)
 
