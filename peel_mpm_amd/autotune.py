@@ -15,10 +15,14 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   bucket in flight);
 * ``const_threads``: work-group size of the per-particle constitutive
   kernel;
-* ``g2p_threads``: work-group size of G2P (one bucket per work-group).
+* ``g2p_threads``: work-group size of G2P (one bucket per work-group);
+* ``cell_block``: the cell block of P2G and of the grid pass's fold -- 4
+  (4x4x4 cells, a 7x7x7-node arena per bucket) or 8 (4x4x8 cells: two
+  buckets adjacent along z share one 7x7x11-node arena, 21 % fewer arena
+  nodes written and folded).
 
-The bucket itself (4x4x4 cells, one cell per lane of a 64-wide wavefront)
-is the unit of the data layout, not a launch parameter.
+The bucket (4x4x4 cells, one cell per lane of a 64-wide wavefront) stays
+the unit of the binning and of the lane-per-cell scatter.
 
 ``tune`` runs a coordinate descent over these knobs on the caller's own
 workload: for each knob in turn it times a few steps at every value (the
@@ -44,9 +48,10 @@ SPACE = {
     "p2g_pair": (1, 2),
     "const_threads": (64, 128, 256),
     "block_wgs": (512, 1024, 2048),
+    "cell_block": (4, 8),
 }
 DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2,
-           "g2p_threads": 256}
+           "g2p_threads": 256, "cell_block": 4}
 _CACHE: dict = {}
 
 

@@ -50,6 +50,8 @@ namespace {
 constexpr int BLK = 4;                 // cells per bucket edge (= nodes per grid-block edge)
 constexpr int AR = BLK + 3;            // arena edge: nodes [4b-1, 4b+5]
 constexpr int AR3 = AR * AR * AR;      // 343
+constexpr int ARZ8 = 2 * BLK + 3;      // z edge of an 8-cell-long block arena: nodes [8c-1, 8c+9]
+constexpr int AR3B = AR * AR * ARZ8;   // 539
 constexpr int WG = 256;                // threads per work-group
 constexpr int PERSIST_WG = 2048;       // work-groups of the persistent bucket kernels
 constexpr int BTILE = WG;              // buckets per bucket-scan tile
@@ -119,7 +121,7 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
-pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256};
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4};
 
 // Workspace layout (byte offsets).  The step keeps count, bflag, mark and
 // misc at zero between steps: whoever reads them clears them.  count/start
@@ -170,7 +172,9 @@ struct Layout {
         pcgd = o;   o = al(o + (size_t)G * 2 * ts);                // PCG damage: per node (Jacobi diagonal, lumped mass)
         gmw = o;    o = al(o + (size_t)G * GMW * ts);              // per node (m, sum w, mv/m) kept for the adjoint
         // ---- sections sized by n
-        arena = o;  o = al(o + (size_t)max_slots * 6 * AR3 * ts);
+        // compact per-bucket arenas (343 nodes), or dense 4x4x8-cell block arenas (539 nodes)
+        const size_t NB8 = (size_t)ns * nblk(ng) * nblk(ng) * ((nblk(ng) + 1) / 2);
+        arena = o;  o = al(o + std::max((size_t)max_slots * AR3, NB8 * AR3B) * 6 * ts);
         key = o;    o = al(o + np * 4);
         slot = o;   o = al(o + np * 4);
         perm = o;   o = al(o + np * 4);
@@ -521,7 +525,9 @@ template <typename T> struct StepArgs {
     unsigned long long* stats;
     // scalars
     long long n, NBs, Gs;
+    long long NBs8;             // 4x4x8-cell blocks per scene (nb x nb x ceil(nb/2))
     int ng, nb, nscenes;
+    int cb;                     // cells along z per P2G block (4 or 8; launch shape)
     const uint8_t* scene_dmg;   // per-scene damage gate, or null: all scenes run damage
     T dx, inv_dx, dt, lim, pad_lo, pad_hi, fc_lo, fc_hi, dt_dmg, damp, stress_k, c_scale, lap_scale;
     T dtg[3];
@@ -980,6 +986,64 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
     }
 }
 
+// One wavefront's share of a bucket's P2G: lane l owns cell l of bucket bk
+// (its particles contiguous in the sorted records) and scatters the cell's
+// particles first, first + step, ... (NP per arena update) into the
+// wavefront's arenas sq, sc, whose node 0 is grid node org.  Waves with on =
+// false only take part in the round count.
+template <typename T, int NP>
+__device__ __forceinline__ void p2g_cells(const StepArgs<T>& A, int bk, const int org[3], int first, int step, bool on,
+                                          T (*sq)[4], T (*sc)[2]) {
+    const int ng = A.ng;
+    const int lane = threadIdx.x & 63;
+    const long long cell = ((long long)bk << 6) + lane;
+    const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
+    int rounds = cnt;
+    for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+    for (int r = first; r < rounds; r += step) {
+        if (!on || r >= cnt) continue;
+        if constexpr (NP == 1) {
+            P2GPart<T> q1[1];
+            int sh1[3], lp1[3][4];
+            p2g_load(A, c0 + r, ng, q1[0], sh1);
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+            p2g_scatter<T, 1>(sq, sc, lp1, q1);
+            continue;
+        }
+        const bool two = r + 1 < cnt;
+        P2GPart<T> q[2];
+        int sh[3];
+        p2g_load(A, c0 + r, ng, q[0], sh);
+        p2g_load(A, c0 + r + (two ? 1 : 0), ng, q[1], sh);
+        // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as
+        // arena positions (not grid-clamped; positions beyond the grid
+        // fold onto the edge node in the grid pass)
+        int lp[3][4];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+        // particles of one cell share the frame unless they lie beyond the
+        // grid (cells clamped together); such a second particle goes alone
+        const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
+        if (!same) p2g_mute(q[1]);
+        p2g_scatter<T, 2>(sq, sc, lp, q);
+        if (two && !same) {
+            P2GPart<T> q1[1];
+            p2g_load(A, c0 + r + 1, ng, q1[0], sh);
+            int lp1[3][4];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+            p2g_scatter<T, 1>(sq, sc, lp1, q1);
+        }
+    }
+}
+
 // P2G (quadratic momentum, cubic damage) per bucket, from the records k_const
 // wrote.  A work-group of P2G_WAVES (2 or 4) wavefronts takes one bucket; in every
 // wavefront lane l owns cell l of the bucket and takes NP of the cell's
@@ -998,8 +1062,8 @@ k_p2g(StepArgs<T> A) {
     __shared__ __align__(16) T s_q[P2G_WAVES][AR3][4];
     __shared__ __align__(16) T s_c[P2G_WAVES][AR3][2];
     const int nact = A.misc[0];
-    const int nb = A.nb, ng = A.ng;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int nb = A.nb;
+    const int wv = threadIdx.x >> 6;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
         const int loc = (int)(bk % A.NBs);
@@ -1008,53 +1072,8 @@ k_p2g(StepArgs<T> A) {
         for (int l = threadIdx.x; l < P2G_WAVES * 4 * AR3; l += NT) (&s_q[0][0][0])[l] = T(0);
         for (int l = threadIdx.x; l < P2G_WAVES * 2 * AR3; l += NT) (&s_c[0][0][0])[l] = T(0);
         __syncthreads();
-        const long long cell = ((long long)bk << 6) + lane;
-        const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
-        int rounds = cnt;
-        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
-        const int wsel = wv < P2G_WAVES ? wv : 0;
-        for (int r = NP * wv; r < rounds; r += NP * P2G_WAVES) {
-            if (wv >= P2G_WAVES || r >= cnt) continue;
-            if constexpr (NP == 1) {
-                P2GPart<T> q1[1];
-                int sh1[3], lp1[3][4];
-                p2g_load(A, c0 + r, ng, q1[0], sh1);
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-                p2g_scatter<T, 1>(s_q[wsel], s_c[wsel], lp1, q1);
-                continue;
-            }
-            const bool two = r + 1 < cnt;
-            P2GPart<T> q[2];
-            int sh[3];
-            p2g_load(A, c0 + r, ng, q[0], sh);
-            p2g_load(A, c0 + r + (two ? 1 : 0), ng, q[1], sh);
-            // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as
-            // arena positions (not grid-clamped; positions beyond the grid
-            // fold onto the edge node in the grid pass)
-            int lp[3][4];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
-            // particles of one cell share the frame unless they lie beyond the
-            // grid (cells clamped together); such a second particle goes alone
-            const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
-            if (!same) p2g_mute(q[1]);
-            p2g_scatter<T, 2>(s_q[wsel], s_c[wsel], lp, q);
-            if (two && !same) {
-                P2GPart<T> q1[1];
-                p2g_load(A, c0 + r + 1, ng, q1[0], sh);
-                int lp1[3][4];
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-                p2g_scatter<T, 1>(s_q[wsel], s_c[wsel], lp1, q1);
-            }
-        }
+        p2g_cells<T, NP>(A, bk, org, NP * wv, NP * P2G_WAVES, wv < P2G_WAVES, s_q[wv < P2G_WAVES ? wv : 0],
+                         s_c[wv < P2G_WAVES ? wv : 0]);
         __syncthreads();
         // sum the wavefront arenas in a fixed order and store this bucket's
         // slot with plain coalesced stores
@@ -1071,16 +1090,71 @@ k_p2g(StepArgs<T> A) {
     }
 }
 
+// P2G with the larger cell block: 4x4x8 cells (two buckets adjacent along z)
+// per work-group, one wavefront per bucket, each with its own 7^3 arena as
+// in k_p2g; the two arenas are summed into one 7x7x11-node block arena and
+// stored at the block's dense index (scene, bx, by, bz / 2) -- the grid pass
+// then folds 8-node-long block arenas along z instead of 4-node ones.  The
+// work-group of a block is the one that meets its even bucket in the active
+// list (or its odd bucket when the even one is empty).
+template <typename T, int NP>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+k_p2g_cb8(StepArgs<T> A) {
+    __shared__ __align__(16) T s_q[2][AR3][4];
+    __shared__ __align__(16) T s_c[2][AR3][2];
+    const int nact = A.misc[0];
+    const int nb = A.nb, nbz = (nb + 1) >> 1;
+    const int wv = threadIdx.x >> 6;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        const int bk = A.alist[it.i];
+        const int scene = (int)(bk / A.NBs), loc = (int)(bk - (long long)scene * A.NBs);
+        const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
+        if ((bz & 1) && A.slotof[bk - 1] >= 0) continue;     // the even bucket's work-group has it
+        const int bz0 = bz & ~1, mbz = bz0 + wv;
+        const bool have = mbz < nb && A.slotof[(long long)scene * A.NBs + (bx * nb + by) * nb + mbz] >= 0;
+        for (int l = threadIdx.x; l < 2 * 4 * AR3; l += 128) (&s_q[0][0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < 2 * 2 * AR3; l += 128) (&s_c[0][0][0])[l] = T(0);
+        __syncthreads();
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * mbz - 1};
+        if (have) p2g_cells<T, NP>(A, bk - bz + mbz, org, 0, NP, true, s_q[wv], s_c[wv]);
+        __syncthreads();
+        // block arena node (lx, ly, lz), lz in [0, 11): bucket 0's lz, bucket 1's lz - 4
+        const long long blk = (long long)scene * A.NBs8 + (bx * nb + by) * nbz + (bz0 >> 1);
+        T* dst = A.arena + blk * (6 * AR3B);
+        for (int l = threadIdx.x; l < 6 * AR3B; l += 128) {
+            const int node = l / 6, c = l - node * 6;
+            const int lxy = node / ARZ8, lz = node - lxy * ARZ8;
+            T v = T(0);
+            if (lz < AR) v += c < 4 ? s_q[0][lxy * AR + lz][c] : s_c[0][lxy * AR + lz][c - 4];
+            if (lz >= BLK) v += c < 4 ? s_q[1][lxy * AR + lz - BLK][c] : s_c[1][lxy * AR + lz - BLK][c - 4];
+            dst[l] = v;
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------ arena fold
-// Arena slot of bucket k = lane of the 27 around block (bx,by,bz), k = 9 dx +
-// 3 dy + dz for bucket offsets dx, dy, dz + 1 in {0,1,2} (-1: outside the
-// grid or empty), on lanes 0..26; -1 on the other lanes.
-template <typename T>
-__device__ __forceinline__ int bucket_slot_lane(const StepArgs<T>& A, int scene, int bx, int by, int bz, int lane) {
+// Arena slot of the arena unit k = lane of the 27 around node block
+// (bx, by, bz), k = 9 dx + 3 dy + dz for unit offsets dx, dy, dz + 1 in
+// {0,1,2}, on lanes 0..26 (-1: outside the grid, empty, or another lane).
+// CB = 1: the units are the 4x4x4-cell buckets, the slot their compact arena
+// slot; CB = 2: along z the units are 4x4x8-cell blocks (uz = bz / 2), the
+// slot their dense block index, present when either bucket holds particles.
+template <typename T, int CB>
+__device__ __forceinline__ int unit_slot_lane(const StepArgs<T>& A, int scene, int bx, int by, int uz, int lane) {
     const int nb = A.nb;
-    const int u = bx - 1 + lane / 9, v = by - 1 + (lane / 3) % 3, w = bz - 1 + lane % 3;
-    const bool in = lane < 27 && u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
-    return in ? A.slotof[scene * A.NBs + (u * nb + v) * nb + w] : -1;
+    const int u = bx - 1 + lane / 9, v = by - 1 + (lane / 3) % 3, w = uz - 1 + lane % 3;
+    if constexpr (CB == 1) {
+        const bool in = lane < 27 && u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nb;
+        return in ? A.slotof[scene * A.NBs + (u * nb + v) * nb + w] : -1;
+    } else {
+        const int nbz = (nb + 1) >> 1;
+        const bool in = lane < 27 && u >= 0 && u < nb && v >= 0 && v < nb && w >= 0 && w < nbz;
+        if (!in) return -1;
+        const long long b0 = scene * A.NBs + (u * nb + v) * nb + 2 * w;
+        const bool act = A.slotof[b0] >= 0 || (2 * w + 1 < nb && A.slotof[b0 + 1] >= 0);
+        return act ? (int)(scene * A.NBs8 + (u * nb + v) * nbz + w) : -1;
+    }
 }
 
 // The first NCH channels of a node's 6 node-major values (8-byte pairs in
@@ -1101,29 +1175,33 @@ template <typename T, int NCH> __device__ __forceinline__ void load_node(const T
 }
 
 // Sum of the first NCH channels of node g over the arenas that hold it, in
-// a fixed bucket order (ascending bucket offsets).  Arena [4b'-1, 4b'+5]
-// holds node 4b+i of block b for b' = b-1 (i = 0, 1), b' = b (all i) and
-// b' = b+1 (i = 3): one or two buckets per axis, at most 8 in all.  The 8
-// candidates are loaded unconditionally -- a missing one reads arena slot 0
-// and is masked to zero -- so all loads are in flight together.  lslot:
-// this wavefront's bucket_slot_lane (every lane of the wavefront must be
-// here).  Edge nodes 0 and ng-1 also collect the arena positions beyond
-// the grid (the frames keep those positions unclamped; the reference clamps
-// such nodes onto the edge node); off_grid: some particle's stencil reached
-// past the grid this step -- otherwise those positions hold zeros.
-template <typename T, int NCH>
+// a fixed unit order (ascending unit offsets).  Per axis an arena unit of U
+// nodes (4, or 8 along z for CB = 2) starting at node U u covers nodes
+// [U u - 1, U u + U + 1]; node U u + i of unit u lies in unit u - 1 too for
+// i = 0, 1 and in unit u + 1 too for i = U - 1: one or two units per axis,
+// at most 8 in all.  The 8 candidates are loaded unconditionally -- a
+// missing one reads slot 0 and is masked to zero -- so all loads are in
+// flight together.  lslot: this wavefront's unit_slot_lane (every lane of
+// the wavefront must be here).  Edge nodes 0 and ng-1 also collect the arena
+// positions beyond the grid (the frames keep those positions unclamped; the
+// reference clamps such nodes onto the edge node); off_grid: some
+// particle's stencil reached past the grid this step -- otherwise those
+// positions hold zeros.
+template <typename T, int NCH, int CB>
 __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int lslot, const int g[3], int bx, int by, int bz,
                                           bool off_grid, T acc[NCH]) {
+    constexpr int UZ = BLK * CB, ARZ = UZ + 3, SLOT = 6 * AR * AR * ARZ;
     const T* __restrict__ arena = A.arena;
     const int ng = A.ng;
-    const int b[3] = {bx, by, bz};
+    const int u[3] = {bx, by, CB == 1 ? bz : bz >> 1};
+    const int U[3] = {BLK, BLK, UZ};
     int lo[3];
     bool two[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int i = g[a] - BLK * b[a];
+        const int i = g[a] - U[a] * u[a];
         lo[a] = i < 2 ? -1 : 0;
-        two[a] = i != 2;
+        two[a] = i < 2 || i == U[a] - 1;
     }
     T v[8][NCH];
 #pragma unroll
@@ -1132,8 +1210,8 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
         const int ux = lo[0] + cx, uy = lo[1] + cy, uz = lo[2] + cz;
         const int s = __shfl(lslot, (ux + 1) * 9 + (uy + 1) * 3 + (uz + 1));
         const bool ok = (cx == 0 || two[0]) && (cy == 0 || two[1]) && (cz == 0 || two[2]) && s >= 0;
-        const int lx = g[0] - (BLK * (bx + ux) - 1), ly = g[1] - (BLK * (by + uy) - 1), lz = g[2] - (BLK * (bz + uz) - 1);
-        const size_t off = ok ? (size_t)s * (6 * AR3) + 6 * ((lx * AR + ly) * AR + lz) : 0;
+        const int lx = g[0] - (BLK * (bx + ux) - 1), ly = g[1] - (BLK * (by + uy) - 1), lz = g[2] - (UZ * (u[2] + uz) - 1);
+        const size_t off = ok ? (size_t)s * SLOT + 6 * ((lx * AR + ly) * ARZ + lz) : 0;
         load_node<T, NCH>(arena + off, v[c]);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) v[c][ch] = ok ? v[c][ch] : T(0);
@@ -1144,35 +1222,42 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
         for (int ch = 0; ch < NCH; ++ch) acc[ch] += v[c][ch];
     const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
     if (!off_grid || !edge) return;
-    const int nb = A.nb;
-    const int* sl = A.slotof + scene * A.NBs;
+    const int nb = A.nb, nbz = CB == 1 ? nb : (nb + 1) >> 1;
+    const int ext[3] = {AR, AR, ARZ};
     for (int k = 0; k < 27; ++k) {
-        const int b3[3] = {bx - 1 + k / 9, by - 1 + (k / 3) % 3, bz - 1 + k % 3};
-        if (b3[0] < 0 || b3[0] >= nb || b3[1] < 0 || b3[1] >= nb || b3[2] < 0 || b3[2] >= nb) continue;
-        const int sk = sl[(b3[0] * nb + b3[1]) * nb + b3[2]];
+        const int b3[3] = {u[0] - 1 + k / 9, u[1] - 1 + (k / 3) % 3, u[2] - 1 + k % 3};
+        if (b3[0] < 0 || b3[0] >= nb || b3[1] < 0 || b3[1] >= nb || b3[2] < 0 || b3[2] >= nbz) continue;
+        long long sk;
+        if constexpr (CB == 1) {
+            sk = A.slotof[scene * A.NBs + (b3[0] * nb + b3[1]) * nb + b3[2]];
+        } else {
+            const long long b0 = scene * A.NBs + (b3[0] * nb + b3[1]) * nb + 2 * b3[2];
+            const bool act = A.slotof[b0] >= 0 || (2 * b3[2] + 1 < nb && A.slotof[b0 + 1] >= 0);
+            sk = act ? scene * A.NBs8 + (b3[0] * nb + b3[1]) * nbz + b3[2] : -1;
+        }
         if (sk < 0) continue;
-        int lo3[3], hi3[3];
+        int lo3[3], hi3[3], own[3];
         bool any_extra = false, ok = true;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const int org = BLK * b3[a] - 1;
+            const int org = U[a] * b3[a] - 1;
             const int plo = g[a] == 0 ? org : g[a];
-            const int phi = g[a] == ng - 1 ? org + AR - 1 : g[a];
+            const int phi = g[a] == ng - 1 ? org + ext[a] - 1 : g[a];
             lo3[a] = (plo > org ? plo : org) - org;
-            hi3[a] = (phi < org + AR - 1 ? phi : org + AR - 1) - org;
+            hi3[a] = (phi < org + ext[a] - 1 ? phi : org + ext[a] - 1) - org;
+            own[a] = g[a] - org;
             ok = ok && lo3[a] <= hi3[a];
-            any_extra = any_extra || lo3[a] != hi3[a] || lo3[a] != g[a] - org;
+            any_extra = any_extra || lo3[a] != hi3[a] || lo3[a] != own[a];
         }
         if (!ok || !any_extra) continue;
-        const T* base = arena + (size_t)sk * (6 * AR3);
+        const T* base = arena + (size_t)sk * SLOT;
         for (int lx = lo3[0]; lx <= hi3[0]; ++lx)
             for (int ly = lo3[1]; ly <= hi3[1]; ++ly)
                 for (int lz = lo3[2]; lz <= hi3[2]; ++lz) {
                     // the node's own position was summed above
-                    if (lx == g[0] - (BLK * b3[0] - 1) && ly == g[1] - (BLK * b3[1] - 1) && lz == g[2] - (BLK * b3[2] - 1))
-                        continue;
+                    if (lx == own[0] && ly == own[1] && lz == own[2]) continue;
                     T e[NCH];
-                    load_node<T, NCH>(base + 6 * ((lx * AR + ly) * AR + lz), e);
+                    load_node<T, NCH>(base + 6 * ((lx * AR + ly) * ARZ + lz), e);
 #pragma unroll
                     for (int ch = 0; ch < NCH; ++ch) acc[ch] += e[ch];
                 }
@@ -1205,7 +1290,7 @@ __device__ __forceinline__ bool collider_setup(const pmpm_collider& cl, const T 
 // (bx,by,bz) lie in the arenas [4b-1, 4b+5] of buckets b = bx-1..bx+1 per
 // axis; a node sums the arenas that cover it, in a fixed bucket order, so
 // the loop over the 27 candidate buckets is uniform across the wavefront.
-template <typename T>
+template <typename T, int CB>
 __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
     __shared__ pmpm_collider s_col[PMPM_MAX_COLLIDERS];
     if (threadIdx.x == 0) {
@@ -1226,7 +1311,8 @@ __global__ void __launch_bounds__(WG) k_grid(StepArgs<T> A) {
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         const int g[3] = {BLK * bx + (lane >> 4), BLK * by + ((lane >> 2) & 3), BLK * bz + (lane & 3)};
         T acc[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
-        fold_node<T, 6>(A, scene, bucket_slot_lane(A, scene, bx, by, bz, lane), g, bx, by, bz, off_grid, acc);
+        fold_node<T, 6, CB>(A, scene, unit_slot_lane<T, CB>(A, scene, bx, by, CB == 1 ? bz : bz >> 1, lane), g, bx, by,
+                            bz, off_grid, acc);
         if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         const int gx = g[0], gy = g[1], gz = g[2];
         const long long node = scene * A.Gs + ((long long)gx * ng + gy) * ng + gz;
@@ -1830,7 +1916,7 @@ __global__ void __launch_bounds__(WG) k_pcg_fold(StepArgs<T> A) {
         T acc[NCH];
 #pragma unroll
         for (int c = 0; c < NCH; ++c) acc[c] = T(0);
-        fold_node<T, NCH>(A, 0, bucket_slot_lane(A, 0, bx, by, bz, lane), g, bx, by, bz, off_grid, acc);
+        fold_node<T, NCH, 1>(A, 0, unit_slot_lane<T, 1>(A, 0, bx, by, bz, lane), g, bx, by, bz, off_grid, acc);
         if (g[0] >= ng || g[1] >= ng || g[2] >= ng) continue;
         const long long node = ((long long)g[0] * ng + g[1]) * ng + g[2];
         T* v = A.gv + 4 * node;
@@ -2143,6 +2229,12 @@ __global__ void __launch_bounds__(WG) k_stencil(const T* __restrict__ x, long lo
 // ------------------------------------------------------------------ host side
 inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
 
+// The grid pass over the P2G arenas of the launch's cell block (A.cb).
+template <typename T> void launch_grid(const StepArgs<T>& A, int wgs, hipStream_t st) {
+    if (A.cb == 8) hipLaunchKernelGGL((k_grid<T, 2>), dim3(wgs), dim3(WG), 0, st, A);
+    else hipLaunchKernelGGL((k_grid<T, 1>), dim3(wgs), dim3(WG), 0, st, A);
+}
+
 // Constitutive update (particle order) then P2G (bucket order).
 template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
     const int ct = g_launch.const_threads;
@@ -2151,6 +2243,11 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
     else if (damage) hipLaunchKernelGGL((k_const<T, true, false>), PB, B, 0, st, A);
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
     const dim3 G(g_launch.bucket_wgs);
+    if (A.cb == 8) {
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_cb8<T, 1>), G, dim3(128), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g_cb8<T, 2>), G, dim3(128), 0, st, A);
+        return;
+    }
     // four fp64 arenas would exceed 64 KiB of LDS per work-group: fp64 always takes two
     if constexpr (sizeof(T) == 4) {
         if (g_launch.p2g_waves == 4) {
@@ -2236,6 +2333,8 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.n = in->n;
     A.ng = p->n_grid; A.nb = nblk(p->n_grid);
     A.NBs = L.NBs; A.Gs = L.Gs;
+    A.NBs8 = (long long)A.nb * A.nb * ((A.nb + 1) / 2);
+    A.cb = g_launch.cell_block;
     A.nscenes = p->n_scenes > 1 ? p->n_scenes : 1;
     A.scene_dmg = p->scene_damage;
     A.dx = T(p->dx); A.inv_dx = T(p->inv_dx); A.dt = T(p->dt); A.lim = T(p->grid_lim);
@@ -2388,7 +2487,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     }
     {
         TimedScope ts(2, st);
-        hipLaunchKernelGGL(k_grid<T>, dim3(g_launch.block_wgs), dim3(WG), 0, st, A);
+        launch_grid<T>(A, g_launch.block_wgs, st);
     }
     HIPCHK(hipGetLastError());
     return PMPM_OK;
@@ -2420,7 +2519,7 @@ int step_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in, const p
         int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, 1, nullptr, 1, L, ws, st);
         if (rc) return rc;
         if (in->n > 0) launch_transfer<T>(A, p->do_damage != 0, false, st);
-        hipLaunchKernelGGL(k_grid<T>, dim3(GRID_WG_COUNT), dim3(WG), 0, st, A);
+        launch_grid<T>(A, GRID_WG_COUNT, st);
     }
     AdjArgs<T> B;
     B.gx = (const T*)gin->x; B.gv = (const T*)gin->v; B.gF = (const T*)gin->F; B.gC = (const T*)gin->C;
@@ -2504,8 +2603,9 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
 }
 
 int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
-    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256};
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4};
     if (cfg) {
+        if (cfg->cell_block) c.cell_block = cfg->cell_block;
         if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
         if (cfg->block_wgs) c.block_wgs = cfg->block_wgs;
         if (cfg->p2g_pair) c.p2g_pair = cfg->p2g_pair;
@@ -2524,6 +2624,8 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
         return fail(PMPM_EARG, "%sp2g_waves must be 1, 2 or 4: %lld", "", c.p2g_waves);
     if (c.g2p_threads != 128 && c.g2p_threads != 256 && c.g2p_threads != 512)
         return fail(PMPM_EARG, "%sg2p_threads must be 128, 256 or 512: %lld", "", c.g2p_threads);
+    if (c.cell_block != 4 && c.cell_block != 8)
+        return fail(PMPM_EARG, "%scell_block must be 4 or 8: %lld", "", c.cell_block);
     g_launch = c;
     return PMPM_OK;
 }
