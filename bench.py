@@ -61,7 +61,7 @@ PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage so
 # The plain step's pinned launch shape: the autotuner's pick for config1
 # (autotune.tune_solver on this scene), and the shape profiles/ were taken at.
 PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 1,
-                 "g2p_threads": 256}
+                 "g2p_threads": 256, "cell_block": 4}
 
 STEP_B_PER_PARTICLE = 289
 DIAG_B_PER_PARTICLE = 48
