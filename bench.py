@@ -2,15 +2,23 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config1]
                     [--cpu-steps 12] [--no-cpu-baseline] [--no-batched] [--no-adjoint] [--no-pcg]
-                    [--no-config2] [--no-carry] [--no-builders] [--tune | --launch-config JSON]
+                    [--no-config2] [--no-carry] [--no-builders] [--no-substep-loop]
+                    [--tune | --launch-config JSON]
 
-Prints ONE JSON line.  A step is one AnisoMPM.substep of the whole scene
-(binning, stress/damage, P2G, grid update, G2P, advection) on particle state
-already resident in HBM; ``value`` = particles x steps / wall seconds of the
-K timed steps (bracketed by torch.cuda.synchronize()).  The headline runs
-the reference's explicit damage update (the parity mode); the config's
-implicit damage solve (PCG, tol 1e-6, <= 50 iterations) is the ``pcg`` leg,
-with its own rooflines.
+Prints ONE JSON line.  A step is one substep of the whole scene (binning,
+stress/damage, P2G, grid update, G2P, advection) on particle state already
+resident in HBM; ``value`` = particles x steps / wall seconds of the K timed
+steps (bracketed by torch.cuda.synchronize()).  The K steps run as
+AnisoMPM.run_frame calls of 10 substeps (the reference's frame loop,
+peel_test.py:271-272) from the scene's initial state: by default K is the
+config's whole run (500 steps for config1), over which the cost per step
+grows as the slab tears and spreads.  A frame writes the diagnostics
+tau_last, J_last, phi and phi_shear only on the substep whose values it
+leaves (what a caller can read).  ``substep_loop`` times K single substep
+calls that write them every step.  The headline runs the
+reference's explicit damage update (the parity mode); the config's implicit
+damage solve (PCG, tol 1e-6, <= 50 iterations) is the ``pcg`` leg, with its
+own rooflines.
 
 Algorithmic bytes follow one fixed convention (fp32):
   whole step   289 B per particle (27 state values read and written, 18
@@ -63,6 +71,8 @@ PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage so
 PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 1,
                  "g2p_threads": 256, "cell_block": 4}
 
+FRAME = 10                          # substeps per run_frame call in the timed legs
+
 STEP_B_PER_PARTICLE = 289
 DIAG_B_PER_PARTICLE = 48
 ADJ_B_PER_PARTICLE = 834
@@ -91,41 +101,89 @@ def make_scene(name):
     return {"config0": scenes.config0, "config1": scenes.config1, "config2": scenes.config2}[name]()
 
 
-def time_gpu(spec, steps, warmup, tune=True, prep=None, **solver_kw):
-    """Time ``steps`` substeps of the scene (after ``warmup``), then the same
-    number again with live per-group timing.  Returns the solver, wall
-    seconds, ms per launch and launches of each timing group, active nodes,
-    and the launch shape used."""
+def frame_length(steps, frame):
+    """Substeps per run_frame call: the largest divisor of steps that is at
+    most frame, so every timed frame has one length (one captured graph)."""
+    f = max(1, min(frame, steps))
+    while steps % f:
+        f -= 1
+    return f
+
+
+def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=False, **solver_kw):
+    """Time ``steps`` substeps of the scene from its initial state (the
+    config's whole run at the default ``steps``), run as
+    run_frame calls of frame_length(steps, frame) substeps (eager launches;
+    use_graph replays a captured HIP graph per frame, which measured slower
+    at this size), then the same run again with live per-group timing.  The
+    warm-up (whole frames, after any tuning) runs first and the initial
+    state is restored before each timed run, so both time steps 0..steps-1
+    of the scene: its cost grows as the slab tears and spreads.  Returns the
+    solver, wall seconds, ms per launch and launches of each timing group,
+    active nodes (mean over the run), and the launch shape used (with the
+    frame length)."""
     import torch
     from peel_mpm_amd import AnisoMPM, _lib, autotune, scenes
-    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", **solver_kw)
+    sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", use_graph=use_graph, **solver_kw)
     if prep:
         prep(sim)
+    names = ("x", "v", "F", "C", "d", "lap", "D_hist", "tau_last", "J_last", "phi", "phi_shear")
+    initial = {k: getattr(sim, k).clone() for k in names if hasattr(sim, k)}
+
+    def restart():
+        for k, t in initial.items():
+            setattr(sim, k, t.clone())
+        sim._step = 0
+
     L = _lib.lib()
     t_tune = time.perf_counter()
     launch = autotune.tune_solver(sim, steps=10) if tune else autotune.get_config()
     t_tune = time.perf_counter() - t_tune
-    for s in range(warmup):
-        sim.substep(s * spec["dt"])
+    f = frame_length(steps, frame)
+    dt = spec["dt"]
+
+    def run(k):
+        for s in range(0, k, f):
+            sim.run_frame(f, s * dt)
+
+    run(-(-max(warmup, 1) // f) * f)
+    restart()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for s in range(steps):
-        sim.substep((warmup + s) * spec["dt"])
+    run(steps)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    # per-group live timing over a second pass of the same number of steps
+    # per-group live timing over the same run, eager (HIP events are not
+    # recorded inside a replayed graph), and the active nodes of each
+    # frame's last substep, averaged over the run
+    restart()
+    sim.use_graph = False
     L.pmpm_timing(1)
-    for s in range(steps):
-        sim.substep((warmup + steps + s) * spec["dt"])
+    qs, cs = [], []
+    for s in range(0, steps, f):
+        sim.run_frame(f, s * dt)
+        q, c = sim.last_step_stats()
+        qs.append(q)
+        cs.append(c)
     torch.cuda.synchronize()
+    n_q, n_c = round(sum(qs) / len(qs)), round(sum(cs) / len(cs))
     ms = (C.c_double * 6)()
     cnt = (C.c_int64 * 6)()
     _lib.check(L.pmpm_timing_read(ms, cnt, 6), "pmpm_timing_read")
     L.pmpm_timing(0)
-    n_q, n_c = sim.last_step_stats()
+    sim.use_graph = use_graph
     groups = {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS + ("pcg_solve", "pcg_iteration"))}
+    graphed = bool(use_graph and sim._graph_eligible())
     return sim, wall, groups, (n_q, n_c), {"config": launch, "source": "autotuned" if tune else "pinned",
-                                           "tuning_s": round(t_tune, 2)}
+                                           "tuning_s": round(t_tune, 2), "frame": f, "graph": graphed}
+
+
+def time_substeps(spec, steps, warmup):
+    """The same run as single substep calls (frames of one substep, so every
+    step writes the diagnostics): ms per step and per-group times."""
+    sim, wall, groups, _, _ = time_gpu(spec, steps, warmup, tune=False, frame=1)
+    return {"ms_per_step": wall / steps * 1e3, "steps": steps, "value": sim.n * steps / wall,
+            "unit": "particle-steps/s", "kernel_ms": {k: round(groups[k][0] / max(groups[k][1], 1), 5) for k in KERNELS}}
 
 
 def time_batched(n_scenes, steps, warmup):
@@ -537,7 +595,7 @@ def time_cpu(spec, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None, help="default: the workload's configured run (500 for config1)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="config1", choices=("config0", "config1", "config2"))
     ap.add_argument("--cpu-steps", type=int, default=12)
@@ -548,6 +606,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=100)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-pcg", action="store_true")
+    ap.add_argument("--no-substep-loop", action="store_true", help="skip the single-substep-call leg")
     ap.add_argument("--no-carry", action="store_true", help="skip the Gaussian-carry and peel-observable legs")
     ap.add_argument("--no-builders", action="store_true", help="skip the scene-builder and Option A legs")
     ap.add_argument("--tune", action="store_true",
@@ -568,6 +627,8 @@ def main():
     if not tune:
         autotune.set_config(json.loads(args.launch_config) if args.launch_config else PINNED_LAUNCH)
     spec = make_scene(args.workload)
+    if args.steps is None:
+        args.steps = spec["steps"]
     sim, wall, groups, (n_q, n_c), launch = time_gpu(spec, args.steps, args.warmup, tune=tune)
     n = sim.n
     finite = bool(__import__("torch").isfinite(sim.x).all())
@@ -583,9 +644,12 @@ def main():
         "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"{args.workload}: {n} particles, {spec['n_grid']}^3 grid, dt={spec['dt']}, explicit "
-                               "damage update (the reference's; the config's PCG damage solve is the pcg leg)",
+                               "damage update (the reference's; the config's PCG damage solve is the pcg leg), "
+                               f"steps 0..{args.steps - 1} from the initial state as run_frame({launch['frame']}) frames"
+                               f"{' replayed from a HIP graph' if launch['graph'] else ''}, diagnostics written on "
+                               "each frame's last substep",
                    "particles": n, "n_grid": spec["n_grid"], "dt": spec["dt"], "form": "plain single-scene",
-                   "damage": "explicit"},
+                   "damage": "explicit", "frame": launch["frame"]},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "algorithmic_bytes": alg,
                      "launch_ms": per_kernel[dominant],
@@ -597,10 +661,12 @@ def main():
         "launch": launch,
         "finite": finite,
     }
+    if not args.no_substep_loop:
+        out["substep_loop"] = time_substeps(spec, args.steps, args.warmup)
     if args.copy_calibration:
         out["copy_calibration"] = copy_calibration(args.copy_calibration)
     if not args.no_pcg:
-        out["pcg"] = time_pcg(spec, min(args.steps, 20), 2, args.workload)
+        out["pcg"] = time_pcg(spec, args.steps, 2, args.workload)
         out["pcg_stiff"] = time_pcg(spec, 10, 1, f"{args.workload}, eta 1e-4 and l0 = 2 dx", prep=stiff_damage)
     if not args.no_carry:
         out["gaussian_carry"] = time_carry(spec, 4, 20)
@@ -611,12 +677,14 @@ def main():
     if not args.no_config2 and args.workload != "config2":
         import torch
         s2 = make_scene("config2")
-        sim2, wall2, g2, (q2, c2), tuned2 = time_gpu(s2, 10, 2, tune=tune)
+        k2 = s2["steps"]
+        sim2, wall2, g2, (q2, c2), tuned2 = time_gpu(s2, k2, 2, tune=tune)
         n2 = sim2.n
-        out["config2"] = {"workload": f"config2: {n2} particles, 256^3 grid, two slabs, dt=2e-05, explicit damage",
-                          "value": n2 * 10 / wall2, "unit": "particle-steps/s", "ms_per_step": wall2 / 10 * 1e3,
+        out["config2"] = {"workload": f"config2: {n2} particles, 256^3 grid, two slabs, dt=2e-05, explicit damage, "
+                                      f"its {k2}-step run as run_frame({tuned2['frame']}) frames",
+                          "value": n2 * k2 / wall2, "unit": "particle-steps/s", "ms_per_step": wall2 / k2 * 1e3,
                           "kernel_ms": {k: round(g2[k][0] / max(g2[k][1], 1), 5) for k in KERNELS},
-                          "step_roofline": step_roofline(step_bytes(n2, q2, c2), wall2 / 10),
+                          "step_roofline": step_roofline(step_bytes(n2, q2, c2), wall2 / k2),
                           "active_nodes": {"quadratic": q2, "cubic": c2},
                           "finite": bool(torch.isfinite(sim2.x).all()), "launch": tuned2}
         del s2, sim2

@@ -136,9 +136,15 @@ typedef struct {
     const uint8_t *bc_id;
 } pmpm_particles_in;
 
-/* Outputs.  x, v, F, C, lap, tau_last, J_last are always written.  d, D_hist,
- * phi, phi_shear are written only when do_damage (the reference then rebinds
- * them); otherwise they may be NULL.  Outputs must not alias inputs. */
+/* Outputs.  x, v, F, C, lap are always written, and so are tau_last and
+ * J_last unless both are NULL.  d, D_hist, phi, phi_shear are written only
+ * when do_damage (the reference then rebinds them, anisompm.py:665-685);
+ * otherwise they may be NULL, and phi and phi_shear may be NULL together
+ * also when do_damage.  The four diagnostics (tau_last, J_last, phi,
+ * phi_shear; anisompm.py:665, :673, :705-706) feed nothing else in the step:
+ * a caller that reads them only after the last substep of a loop passes
+ * NULL on the others and saves their 48 bytes per particle of stores.
+ * Outputs must not alias inputs. */
 typedef struct {
     void *x, *v, *F, *C, *lap, *d, *D_hist, *phi, *phi_shear, *tau_last, *J_last;
 } pmpm_particles_out;

@@ -801,9 +801,11 @@ __device__ __forceinline__ M3<T> constitutive(const StepArgs<T>& A, long long p,
                 for (int i = 0; i < 3; ++i) { T w = Sn[i] - tn * n0[i]; tt2 += w * w; }
             }
         }
-        T tt = sqrt(tt2) / sc;
-        A.ophi[p] = phi;
-        A.ophs[p] = tt * tt;
+        if (A.ophi) {               // null: a frame's caller reads these after a later substep
+            const T tt = sqrt(tt2) / sc;
+            A.ophi[p] = phi;
+            A.ophs[p] = tt * tt;
+        }
         // history max of the drive, explicit phase-field update (anisompm.py:679-685)
         const T Dh = maxp(q.Dh, q.zeta * relu(phi - T(1)));
         A.oD[p] = Dh;
@@ -847,8 +849,10 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     T d = q.d, J;
     M3<T> tau = constitutive<T, DAMAGE, EXACT_SIGMA>(A, p, q, F, d, J, dmg_on);
     if (A.use_fclamp) store_m3(A.oF, p, F);
-    store_m3(A.otau, p, tau);
-    A.oJ[p] = J;
+    if (A.otau) {
+        store_m3(A.otau, p, tau);
+        A.oJ[p] = J;
+    }
     T sk = T(-1) * A.dt * vol0;
     sk = sk * T(4);
     sk = sk * A.inv_dx;
@@ -986,6 +990,53 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
     }
 }
 
+// One pass of a lane over its cell: the sorted particle record i (and i + 1
+// when NP is 2 and left > 1, left being the cell's records from i on)
+// scattered into the arenas sq, sc, whose node 0 is grid node org.
+template <typename T, int NP>
+__device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2]) {
+    const int ng = A.ng;
+    if constexpr (NP == 1) {
+        P2GPart<T> q1[1];
+        int sh1[3], lp1[3][4];
+        p2g_load(A, i, ng, q1[0], sh1);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+        p2g_scatter<T, 1>(sq, sc, lp1, q1);
+        return;
+    }
+    const bool two = left > 1;
+    P2GPart<T> q[2];
+    int sh[3];
+    p2g_load(A, i, ng, q[0], sh);
+    p2g_load(A, i + (two ? 1 : 0), ng, q[1], sh);
+    // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as arena
+    // positions (not grid-clamped; positions beyond the grid fold onto the
+    // edge node in the grid pass)
+    int lp[3][4];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+    // particles of one cell share the frame unless they lie beyond the grid
+    // (cells clamped together); such a second particle goes alone
+    const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
+    if (!same) p2g_mute(q[1]);
+    p2g_scatter<T, 2>(sq, sc, lp, q);
+    if (two && !same) {
+        P2GPart<T> q1[1];
+        p2g_load(A, i + 1, ng, q1[0], sh);
+        int lp1[3][4];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
+        p2g_scatter<T, 1>(sq, sc, lp1, q1);
+    }
+}
+
 // One wavefront's share of a bucket's P2G: lane l owns cell l of bucket bk
 // (its particles contiguous in the sorted records) and scatters the cell's
 // particles first, first + step, ... (NP per arena update) into the
@@ -994,7 +1045,6 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 template <typename T, int NP>
 __device__ __forceinline__ void p2g_cells(const StepArgs<T>& A, int bk, const int org[3], int first, int step, bool on,
                                           T (*sq)[4], T (*sc)[2]) {
-    const int ng = A.ng;
     const int lane = threadIdx.x & 63;
     const long long cell = ((long long)bk << 6) + lane;
     const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
@@ -1002,45 +1052,7 @@ __device__ __forceinline__ void p2g_cells(const StepArgs<T>& A, int bk, const in
     for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
     for (int r = first; r < rounds; r += step) {
         if (!on || r >= cnt) continue;
-        if constexpr (NP == 1) {
-            P2GPart<T> q1[1];
-            int sh1[3], lp1[3][4];
-            p2g_load(A, c0 + r, ng, q1[0], sh1);
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-            p2g_scatter<T, 1>(sq, sc, lp1, q1);
-            continue;
-        }
-        const bool two = r + 1 < cnt;
-        P2GPart<T> q[2];
-        int sh[3];
-        p2g_load(A, c0 + r, ng, q[0], sh);
-        p2g_load(A, c0 + r + (two ? 1 : 0), ng, q[1], sh);
-        // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as
-        // arena positions (not grid-clamped; positions beyond the grid
-        // fold onto the edge node in the grid pass)
-        int lp[3][4];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
-        // particles of one cell share the frame unless they lie beyond the
-        // grid (cells clamped together); such a second particle goes alone
-        const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
-        if (!same) p2g_mute(q[1]);
-        p2g_scatter<T, 2>(sq, sc, lp, q);
-        if (two && !same) {
-            P2GPart<T> q1[1];
-            p2g_load(A, c0 + r + 1, ng, q1[0], sh);
-            int lp1[3][4];
-#pragma unroll
-            for (int a = 0; a < 3; ++a)
-#pragma unroll
-                for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-            p2g_scatter<T, 1>(sq, sc, lp1, q1);
-        }
+        p2g_pass<T, NP>(A, c0 + r, cnt - r, org, sq, sc);
     }
 }
 
@@ -2382,16 +2394,17 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     }
     if (in->n > 0) {
         const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, in->vol0, in->mass, in->mu, in->lam,
-                              in->residual, in->interface_mask, out->x, out->v, out->F, out->C, out->lap, out->tau_last,
-                              out->J_last};
+                              in->residual, in->interface_mask, out->x, out->v, out->F, out->C, out->lap};
         for (const void* q : need)
             if (!q) return fail(PMPM_EARG, "%smissing particle array", "");
         if (in->n_interface == nullptr) return fail(PMPM_EARG, "%smissing n_interface", "");
+        if (!out->tau_last != !out->J_last) return fail(PMPM_EARG, "%stau_last and J_last are skipped together", "");
         if (p->do_damage) {
             const void* dm[] = {in->D_hist, in->sigma_c, in->eta, in->zeta, in->l0, in->A0, in->allow,
-                                out->d, out->D_hist, out->phi, out->phi_shear};
+                                out->d, out->D_hist};
             for (const void* q : dm)
                 if (!q) return fail(PMPM_EARG, "%smissing damage array", "");
+            if (!out->phi != !out->phi_shear) return fail(PMPM_EARG, "%sphi and phi_shear are skipped together", "");
         }
         if (out->x == in->x || out->F == in->F || out->v == in->v || out->C == in->C)
             return fail(PMPM_EARG, "%soutputs must not alias inputs", "");

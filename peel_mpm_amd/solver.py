@@ -20,7 +20,14 @@ Differences a caller can observe:
   second Jacobi.  The two agree to float rounding;
 * ``damage_solver="pcg"`` (not in the reference) replaces the explicit
   damage update by an implicit grid solve (include/pmpm.h,
-  PMPM_DAMAGE_PCG); the default "explicit" is the reference's update.
+  PMPM_DAMAGE_PCG); the default "explicit" is the reference's update;
+* ``run_frame`` writes the diagnostics ``tau_last``, ``J_last``, ``phi``
+  and ``phi_shear`` (anisompm.py:665, :673, :705-706) only on the substep
+  whose values the reference leaves in them after the frame (the last one;
+  for ``phi`` and ``phi_shear`` the last that updates damage), when no host
+  callable runs inside the frame to see the others.  The state after the
+  frame is the same; ``frame_diagnostics="every"`` writes them on every
+  substep.
 """
 from __future__ import annotations
 
@@ -47,7 +54,7 @@ class AnisoMPM:
                  grid_damp: float = 1.0, f_clamp=None, damage_every: int = 1, device: str = "cuda:0",
                  dtype=torch.float32, exact_sigma_eig: bool = False, stable_sort: bool = True,
                  damage_solver: str = "explicit", pcg_tol: float = 1e-6, pcg_max_iter: int = 50,
-                 use_graph: bool = False):
+                 use_graph: bool = False, frame_diagnostics: str = "end"):
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the native AnisoMPM step runs on a ROCm GPU (device='cuda:N')")
@@ -84,6 +91,9 @@ class AnisoMPM:
         self._bc_id = None
         self._p, self._p_key = None, None
         self.use_graph = bool(use_graph)
+        if frame_diagnostics not in ("end", "every"):
+            raise ValueError("frame_diagnostics is 'end' (written on the frame's last substep) or 'every'")
+        self.frame_diagnostics = frame_diagnostics
         self._graph = None
         self._pin = _lib.ParticlesIn()
         self._out_n = -1
@@ -271,10 +281,12 @@ class AnisoMPM:
         self._out_parts, self._out_parts_dmg = lay(base), lay(base + extra)
         self._out_n = n
 
-    def _prepare(self, time):
+    def _prepare(self, time, diag=True, phi=True):
         """Everything one substep hands to libpmpm: (params, particles in,
         particles out, the new tensors, whether colliders run on the
-        device).  Runs the plain-callable particle_bc hooks."""
+        device).  Runs the plain-callable particle_bc hooks.  diag=False
+        skips tau_last and J_last, phi=False phi and phi_shear (the
+        attributes keep their tensors)."""
         bc_id, bc_vals = self._velocity_bcs(time)
         if bc_id is None:
             for bc in self.particle_bc:
@@ -309,14 +321,18 @@ class AnisoMPM:
         if self._out_n != self.n:
             self._layout_outputs()
         new = self._outputs(do_damage)
+        if not diag:
+            del new["tau_last"], new["J_last"]
+        if do_damage and not phi:
+            del new["phi"], new["phi_shear"]
         pout = _lib.ParticlesOut()
         for k, t in new.items():
             setattr(pout, k, t.data_ptr())
         return p, pin, pout, new, device_cols
 
-    def substep(self, time: float = 0.0):
+    def substep(self, time: float = 0.0, _diag: bool = True, _phi: bool = True):
         L = _lib.lib()
-        p, pin, pout, new, device_cols = self._prepare(time)
+        p, pin, pout, new, device_cols = self._prepare(time, _diag, _phi)
         ws = self._workspace()
         st = _lib.stream_ptr(self.device)
         args = (C.byref(p), C.byref(pin), C.byref(pout), ws.data_ptr(), ws.numel(), st)
@@ -355,19 +371,39 @@ class AnisoMPM:
         use_graph=True and a frame the device can run on its own (every
         particle_bc a VelocityBC, every collider a device collider, the
         explicit damage update), the n_sub substeps are captured once as one
-        HIP graph and replayed: see _run_frame_graph."""
+        HIP graph and replayed: see _run_frame_graph.  In a frame with no
+        host callable the diagnostics are written only where their final
+        values come from (_diag_substeps); the state and the diagnostics the
+        frame leaves are the same as with frame_diagnostics="every"."""
         if n_sub > 0 and self.use_graph and self._graph_eligible():
             self._run_frame_graph(int(n_sub), float(time0))
             return
+        last, last_dmg = self._diag_substeps(n_sub)
         for s in range(n_sub):
-            self.substep(time0 + s * self.dt)
+            self.substep(time0 + s * self.dt, _diag=last is None or s == last, _phi=last is None or s == last_dmg)
+
+    def _device_frame(self):
+        """Whether a frame runs no host callable (every particle_bc a
+        VelocityBC, every collider a device collider)."""
+        return (len(self.colliders) <= _lib.MAX_COLLIDERS and len(self.particle_bc) <= _lib.MAX_VEL_BC
+                and all(isinstance(b, VelocityBC) for b in self.particle_bc)
+                and all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders))
+
+    def _diag_substeps(self, n_sub):
+        """(last substep, last damage-updating substep) of a frame starting at
+        self._step: the substeps that write tau_last/J_last and phi/phi_shear
+        when only the frame's final values are seen; (None, None) when every
+        substep writes them (frame_diagnostics="every", or a host callable
+        runs inside the frame).  The last damage substep is -1 when none
+        updates damage."""
+        if self.frame_diagnostics == "every" or not self._device_frame():
+            return None, None
+        dmg = [s for s in range(n_sub) if (self._step + s) % self.damage_every == 0]
+        return n_sub - 1, (dmg[-1] if dmg else -1)
 
     # ------------------------------------------------------------------ captured frames
     def _graph_eligible(self):
-        return (self.damage_solver == "explicit" and len(self.colliders) <= _lib.MAX_COLLIDERS
-                and len(self.particle_bc) <= _lib.MAX_VEL_BC
-                and all(isinstance(b, VelocityBC) for b in self.particle_bc)
-                and all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders))
+        return self.damage_solver == "explicit" and self._device_frame()
 
     def _run_frame_graph(self, n_sub, time0):
         """One frame of n_sub substeps replayed from a captured HIP graph.
@@ -418,12 +454,12 @@ class AnisoMPM:
             G["pattern"][pk] = torch.tensor([(self._step + s) % self.damage_every == 0 for s in range(n_sub)],
                                             dtype=torch.uint8).to(dev)
         torch.mul(G["pattern"][pk], allow.any().to(torch.uint8), out=G["gate"][:n_sub])
-        key = (n_sub, cur, self._config_key(), len(self.colliders), len(self.particle_bc),
+        key = (n_sub, cur, self._diag_substeps(n_sub), self._config_key(), len(self.colliders), len(self.particle_bc),
                bc_id.data_ptr() if bc_id is not None else 0, allow.data_ptr(), imask.data_ptr(), ws.data_ptr(),
                tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS), tuple(_launch_shape()))
         g = G["graphs"].get(key)
         if g is None:
-            G["graphs"] = {k2: v for k2, v in G["graphs"].items() if k2[2:] == key[2:]}
+            G["graphs"] = {k2: v for k2, v in G["graphs"].items() if k2[3:] == key[3:]}
             launches = self._frame_launches(n_sub, cur, bc_id, allow, imask, ws)
             s = torch.cuda.Stream(device=dev)
             s.wait_stream(torch.cuda.current_stream(dev))
@@ -451,6 +487,7 @@ class AnisoMPM:
         L = _lib.lib()
         G = self._graph
         base = self._step_params()
+        last, last_dmg = self._diag_substeps(n_sub)
         out = []
         for s in range(n_sub):
             src, dst = G["bufs"][(cur + s) % 2], G["bufs"][(cur + s + 1) % 2]
@@ -474,7 +511,8 @@ class AnisoMPM:
             for k in _STATE:
                 setattr(pout, k, dst[k].data_ptr())
             for k, t in G["diag"].items():
-                setattr(pout, k, t.data_ptr())
+                keep = last is None or s == (last if k in ("tau_last", "J_last") else last_dmg)
+                setattr(pout, k, t.data_ptr() if keep else None)
 
             def launch(p=p, pin=pin, pout=pout):
                 _lib.check(L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), ws.data_ptr(), ws.numel(),

@@ -79,6 +79,34 @@ def test_workspace_not_set_up_is_refused_before_any_launch():
     assert L.pmpm_workspace_init(fake, 1 << 30, -1, 32, 1, _lib.PMPM_F32, None) == -1
 
 
+def test_diagnostic_outputs_are_skipped_in_pairs():
+    """tau_last/J_last, and phi/phi_shear when damage runs, may be NULL only
+    together; a half-skipped pair is refused on the host (every case below
+    fails there, so the fake addresses are never dereferenced)."""
+    from peel_mpm_amd import _lib
+    L = _lib.lib()
+    fake = 0x7F0000000000
+    p, pin, pout = _lib.StepParams(), _lib.ParticlesIn(), _lib.ParticlesOut()
+    p.dtype, p.n_grid = _lib.PMPM_F32, 32
+    pin.n = 10
+    for name, _ in _lib.ParticlesIn._fields_:
+        if name != "n" and name != "bc_id":
+            setattr(pin, name, fake)
+    for name, _ in _lib.ParticlesOut._fields_:
+        setattr(pout, name, fake + 4096)
+    pout.J_last = None
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 1 << 40, None)
+    assert rc == -1 and b"tau_last and J_last" in L.pmpm_last_error()
+    pout.tau_last, pout.J_last = None, fake + 4096
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 1 << 40, None)
+    assert rc == -1 and b"tau_last and J_last" in L.pmpm_last_error()
+    pout.J_last = None
+    p.do_damage = 1
+    pout.phi_shear = None
+    rc = L.pmpm_substep(C.byref(p), C.byref(pin), C.byref(pout), None, 1 << 40, None)
+    assert rc == -1 and b"phi and phi_shear" in L.pmpm_last_error()
+
+
 def test_product_refuses_cpu_device():
     import torch
     from peel_mpm_amd import AnisoMPM

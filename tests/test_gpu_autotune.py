@@ -49,34 +49,70 @@ def test_single_particle_p2g_matches_paired_to_rounding():
             assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, (cfg, k)
 
 
+LAYOUTS = [{"cell_block": 8}]
+
+
+def uneven_spec(dtype, seed=5):
+    """A slab whose particles are redrawn uniformly over its box (Poisson
+    cell counts, empty cells among full ones) with 40 of them piled into one
+    cell and 12 into a cell of the next bucket along z: lanes of a packed
+    wavefront then hold several cells of unequal load.  The pile drives
+    many particles to the damage threshold, where the last bit of a stress
+    decides the relu of the update; so damage is frozen (allow off) at a
+    random field, which the cubic scatter and the Laplacian still carry."""
+    import numpy as np
+    from peel_mpm_amd import scenes
+    spec = scenes.slab_block(n_grid=32, cells=(12, 5, 14), origin=(9, 9, 7), ppc=6, seed=seed, sigma_c=20.0,
+                             dtype=dtype)
+    x = spec["arrays"]["x"]
+    rng = np.random.default_rng(seed)
+    lo, hi = x.min(0), x.max(0)
+    x[:] = lo + (hi - lo) * rng.random(x.shape)
+    dx = 1.0 / 32
+    x[:40] = (np.array([13.0, 11.0, 10.0]) + 0.1 + 0.8 * rng.random((40, 3))) * dx
+    x[40:52] = (np.array([13.0, 11.0, 14.0]) + 0.1 + 0.8 * rng.random((12, 3))) * dx
+    spec["arrays"]["d"] = 0.5 * rng.random(x.shape[0])
+    spec["arrays"]["allow"] = np.zeros(x.shape[0], dtype=bool)
+    return spec
+
+
+@pytest.mark.parametrize("cfg", LAYOUTS, ids=lambda c: "-".join(f"{k}{v}" for k, v in c.items()))
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
-def test_cell_block_8_matches_4_to_rounding(dtype):
-    """The 4x4x8-cell P2G block (two buckets' arenas summed into one, folded
-    by the grid pass) against the 4x4x4 one: the same sums in another order.
-    Grids of 8 and 9 buckets per edge (an odd count leaves the last block
-    half empty), explicit and PCG damage."""
+def test_block_layouts_match_the_4x4x4_bucket_to_rounding(cfg, dtype):
+    """The other P2G layouts against one 4x4x4-cell bucket per wavefront:
+    the 4x4x8-cell block (two buckets' arenas summed into one, folded by the
+    grid pass) and two or four buckets packed onto one wavefront's lanes by
+    load -- the same sums in another order.  Grids of 8 and 9 buckets per
+    edge (an odd count leaves the last block or group half empty), a slab
+    with uneven cells, explicit and PCG damage."""
     from peel_mpm_amd import scenes
     tol = 1e-6 if dtype == torch.float32 else 1e-12
-    for n_grid, cells, origin in ((32, (16, 4, 16), (8, 10, 8)), (36, (20, 5, 18), (6, 9, 7))):
-        spec = scenes.slab_block(n_grid=n_grid, cells=cells, origin=origin, ppc=8, seed=3, sigma_c=20.0, dtype=dtype)
+    cases = [(32, (16, 4, 16), (8, 10, 8)), (36, (20, 5, 18), (6, 9, 7)), "uneven"]
+    for case in cases:
+        if case == "uneven":
+            spec = uneven_spec(dtype)
+        else:
+            n_grid, cells, origin = case
+            spec = scenes.slab_block(n_grid=n_grid, cells=cells, origin=origin, ppc=8, seed=3, sigma_c=20.0, dtype=dtype)
         for solver in ("explicit", "pcg"):
-            print(f"cell_block 8 vs 4: n_grid {n_grid} {solver} {dtype}", flush=True)
+            print(f"{cfg} vs 4x4x4: {case} {solver} {dtype}", flush=True)
             a = run({"cell_block": 4}, spec=spec, damage_solver=solver)
-            b = run({"cell_block": 8}, spec=spec, damage_solver=solver)
+            b = run(cfg, spec=spec, damage_solver=solver)
             for k in ("x", "v", "F", "C", "d", "lap"):
                 e = rel_err(getattr(b, k), getattr(a, k))
-                assert e <= (tol if k not in ("d", "lap") else 100 * tol), (n_grid, solver, k, e)
+                assert e <= (tol if k not in ("d", "lap") else 100 * tol), (case, solver, k, e)
 
 
-def test_cell_block_8_at_the_domain_edges_matches_the_golden():
+@pytest.mark.parametrize("cfg", LAYOUTS, ids=lambda c: "-".join(f"{k}{v}" for k, v in c.items()))
+def test_block_layouts_at_the_domain_edges_match_the_golden(cfg):
     """The reference's edge case (particles near and past the domain faces:
     clamp-to-edge node indices, arena positions beyond the grid folded onto
-    the edge nodes) with cell_block 8, against the reference's fp32 output."""
+    the edge nodes) with each layout, against the reference's fp32 output."""
     from peel_mpm_amd import autotune
     from tests.golden_cases import load_npz
     from tests.test_gpu_parity import gpu_sim_from_case
     rec = load_npz("step_edges32.npz")
-    autotune.set_config({"cell_block": 8})
+    autotune.set_config(cfg)
     try:
         sim, time0, nsteps = gpu_sim_from_case(rec)
         for s in range(nsteps):
@@ -89,12 +125,15 @@ def test_cell_block_8_at_the_domain_edges_matches_the_golden():
         assert e <= 1e-5, (k, e)
 
 
-def test_batched_cell_block_8_matches_plain_bit_for_bit():
+@pytest.mark.parametrize("cfg", LAYOUTS, ids=lambda c: "-".join(f"{k}{v}" for k, v in c.items()))
+def test_batched_block_layouts_match_plain_bit_for_bit(cfg):
+    """Blocks and packed groups are fixed by bucket position within a scene,
+    so a scene's sums run in the same order batched or alone."""
     from peel_mpm_amd import AnisoMPM, VelocityBC, autotune, scenes
     from peel_mpm_amd.batched import BatchedAnisoMPM
-    autotune.set_config({"cell_block": 8})
+    autotune.set_config(cfg)
     try:
-        specs = [scenes.slab_block(n_grid=36, cells=(6 + 3 * s, 4, 9 - s), origin=(8, 8 + s, 9), ppc=8, seed=s)
+        specs = [scenes.slab_block(n_grid=36, cells=(6 + 3 * s, 4 + s, 9 - s), origin=(8, 8 + s, 9), ppc=8, seed=s)
                  for s in range(3)]
         b = BatchedAnisoMPM(n_grid=36, dt=1e-4, device=DEV, use_graph=True)
         for sp in specs:
