@@ -932,11 +932,11 @@ template <typename T> __device__ __forceinline__ void p2g_mute(P2GPart<T>& q) {
 // write (so after it, in LDS order), and the next node's contributions are
 // computed while that read is in flight; a scheduling barrier keeps that
 // arithmetic behind the read.
-template <typename T, int NP>
+template <typename T, int NP, int AZ = AR>
 __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp[3][4], const P2GPart<T> (&q)[NP]) {
     T c0, c1, c2, c3, f0, f1;
     {
-        const int l = (lp[0][0] * AR + lp[1][0]) * AR + lp[2][0];
+        const int l = (lp[0][0] * AR + lp[1][0]) * AZ + lp[2][0];
         c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3];
         f0 = sc[l][0]; f1 = sc[l][1];
     }
@@ -960,7 +960,7 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
                 CW[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
                 CD[j] = CW[j] * q[j].d;
             }
-            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
+            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AZ;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
 #pragma unroll
@@ -980,7 +980,7 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
                 if (n < 64) {
                     const int nx = n >> 4, ny = (n >> 2) & 3, nz = n & 3;
-                    const int ln = (lp[0][nx] * AR + lp[1][ny]) * AR + lp[2][nz];
+                    const int ln = (lp[0][nx] * AR + lp[1][ny]) * AZ + lp[2][nz];
                     c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3];
                     f0 = sc[ln][0]; f1 = sc[ln][1];
                 }
@@ -993,7 +993,7 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 // One pass of a lane over its cell: the sorted particle record i (and i + 1
 // when NP is 2 and left > 1, left being the cell's records from i on)
 // scattered into the arenas sq, sc, whose node 0 is grid node org.
-template <typename T, int NP>
+template <typename T, int NP, int AZ = AR>
 __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2]) {
     const int ng = A.ng;
     if constexpr (NP == 1) {
@@ -1003,8 +1003,8 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-        p2g_scatter<T, 1>(sq, sc, lp1, q1);
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
+        p2g_scatter<T, 1, AZ>(sq, sc, lp1, q1);
         return;
     }
     const bool two = left > 1;
@@ -1019,12 +1019,12 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
     // particles of one cell share the frame unless they lie beyond the grid
     // (cells clamped together); such a second particle goes alone
     const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
     if (!same) p2g_mute(q[1]);
-    p2g_scatter<T, 2>(sq, sc, lp, q);
+    p2g_scatter<T, 2, AZ>(sq, sc, lp, q);
     if (two && !same) {
         P2GPart<T> q1[1];
         p2g_load(A, i + 1, ng, q1[0], sh);
@@ -1032,8 +1032,8 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-        p2g_scatter<T, 1>(sq, sc, lp1, q1);
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
+        p2g_scatter<T, 1, AZ>(sq, sc, lp1, q1);
     }
 }
 
@@ -1102,44 +1102,105 @@ k_p2g(StepArgs<T> A) {
     }
 }
 
-// P2G with the larger cell block: 4x4x8 cells (two buckets adjacent along z)
-// per work-group, one wavefront per bucket, each with its own 7^3 arena as
-// in k_p2g; the two arenas are summed into one 7x7x11-node block arena and
-// stored at the block's dense index (scene, bx, by, bz / 2) -- the grid pass
-// then folds 8-node-long block arenas along z instead of 4-node ones.  The
-// work-group of a block is the one that meets its even bucket in the active
-// list (or its odd bucket when the even one is empty).
+// P2G with the larger cell block: 4x4x8 cells (the two buckets adjacent
+// along z) per wavefront, into one 7x7x11-node block arena stored at the
+// block's dense index (scene, bx, by, bz / 2) -- the grid pass then folds
+// 8-node-long block arenas along z instead of 4-node ones.  The block's 128
+// (bucket, cell) items go to the 64 lanes packed by load: ranked by pair
+// passes (ties by item index, so the order is fixed) and dealt in snake
+// order (lane l takes ranks l and 127 - l), each lane running its items'
+// passes back to back.  A thin or torn body leaves buckets part-empty and
+// their cells unevenly filled, which one bucket per wavefront pays as
+// max-cell-count passes with most lanes idle.  An item belongs to one lane,
+// so the lanes of an instruction hold distinct cells of the block and, at a
+// common frame offset, touch distinct nodes, as in k_p2g.  The work-group of
+// a block is the one that meets its even bucket in the active list (or its
+// odd bucket when the even one is empty); blocks are fixed by bucket
+// position within a scene, so a scene's sums run in the same order batched
+// or alone.
 template <typename T, int NP>
-__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 3 : 1)))
 k_p2g_cb8(StepArgs<T> A) {
-    __shared__ __align__(16) T s_q[2][AR3][4];
-    __shared__ __align__(16) T s_c[2][AR3][2];
+    constexpr int NI = 128;
+    __shared__ __align__(16) T s_q[AR3B][4];
+    __shared__ __align__(16) T s_c[AR3B][2];
+    // the item ranking runs before the arena is cleared, in the arena's memory
+    unsigned* s_key = reinterpret_cast<unsigned*>(&s_q[0][0]);
+    int* s_item = reinterpret_cast<int*>(s_key + NI);
+    int* s_c0 = s_item + NI;
+    int* s_n = s_c0 + NI;
     const int nact = A.misc[0];
     const int nb = A.nb, nbz = (nb + 1) >> 1;
-    const int wv = threadIdx.x >> 6;
+    const int lane = threadIdx.x;
     for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
         const int bk = A.alist[it.i];
         const int scene = (int)(bk / A.NBs), loc = (int)(bk - (long long)scene * A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
         if ((bz & 1) && A.slotof[bk - 1] >= 0) continue;     // the even bucket's work-group has it
-        const int bz0 = bz & ~1, mbz = bz0 + wv;
-        const bool have = mbz < nb && A.slotof[(long long)scene * A.NBs + (bx * nb + by) * nb + mbz] >= 0;
-        for (int l = threadIdx.x; l < 2 * 4 * AR3; l += 128) (&s_q[0][0][0])[l] = T(0);
-        for (int l = threadIdx.x; l < 2 * 2 * AR3; l += 128) (&s_c[0][0][0])[l] = T(0);
+        const int bz0 = bz & ~1;
+        const long long b0 = (long long)bk - bz + bz0;         // the block's even bucket
+        // items j * 64 + cell: key = pair passes, then the reverse index
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            int c0 = 0, n = 0;
+            if (bz0 + j < nb && A.slotof[b0 + j] >= 0) {
+                const long long cell = ((b0 + j) << 6) + lane;
+                c0 = A.start[cell];
+                n = A.start[cell + 1] - c0;
+            }
+            const unsigned pp = (unsigned)(NP == 2 ? (n + 1) >> 1 : n);
+            const int idx = j * 64 + lane;
+            s_key[idx] = ((pp < 0xFFFFu ? pp : 0xFFFFu) << 16) | (unsigned)(0xFFFF - idx);
+            s_c0[idx] = c0;
+            s_n[idx] = n;
+        }
         __syncthreads();
-        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * mbz - 1};
-        if (have) p2g_cells<T, NP>(A, bk - bz + mbz, org, 0, NP, true, s_q[wv], s_c[wv]);
+        {
+            // rank = number of larger keys (all keys differ)
+            const unsigned k0 = s_key[lane], k1 = s_key[64 + lane];
+            int r0 = 0, r1 = 0;
+            for (int m = 0; m < NI; m += 4) {
+                const uint4 o = *reinterpret_cast<const uint4*>(&s_key[m]);
+                r0 += (o.x > k0) + (o.y > k0) + (o.z > k0) + (o.w > k0);
+                r1 += (o.x > k1) + (o.y > k1) + (o.z > k1) + (o.w > k1);
+            }
+            s_item[r0] = lane;
+            s_item[r1] = 64 + lane;
+        }
         __syncthreads();
-        // block arena node (lx, ly, lz), lz in [0, 11): bucket 0's lz, bucket 1's lz - 4
+        // this lane's two items (snake order: rank l, then 127 - l; the
+        // empty items rank last, so an empty first item means no second)
+        const int ia = s_item[lane], ib = s_item[127 - lane];
+        int ca = s_c0[ia], na = s_n[ia];
+        int cb = s_c0[ib], nbi = s_n[ib];
+        const int load = (NP == 2 ? (na + 1) >> 1 : na) + (NP == 2 ? (nbi + 1) >> 1 : nbi);
+        int rounds = load;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        __syncthreads();                    // items read: the arena may be cleared
+        for (int l = lane; l < 4 * AR3B; l += 64) (&s_q[0][0])[l] = T(0);
+        for (int l = lane; l < 2 * AR3B; l += 64) (&s_c[0][0])[l] = T(0);
+        __syncthreads();
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz0 - 1};
+        int r = 0;
+        for (int t = 0; t < rounds; ++t) {
+            if (r < na) {
+                p2g_pass<T, NP, ARZ8>(A, ca + r, na - r, org, s_q, s_c);
+                r += NP;
+                if (r >= na) {
+                    ca = cb;
+                    na = nbi;
+                    nbi = 0;
+                    r = 0;
+                }
+            }
+        }
+        __syncthreads();
+        // the block arena, node-major [539][6] at node (lx * 7 + ly) * 11 + lz
         const long long blk = (long long)scene * A.NBs8 + (bx * nb + by) * nbz + (bz0 >> 1);
         T* dst = A.arena + blk * (6 * AR3B);
-        for (int l = threadIdx.x; l < 6 * AR3B; l += 128) {
+        for (int l = lane; l < 6 * AR3B; l += 64) {
             const int node = l / 6, c = l - node * 6;
-            const int lxy = node / ARZ8, lz = node - lxy * ARZ8;
-            T v = T(0);
-            if (lz < AR) v += c < 4 ? s_q[0][lxy * AR + lz][c] : s_c[0][lxy * AR + lz][c - 4];
-            if (lz >= BLK) v += c < 4 ? s_q[1][lxy * AR + lz - BLK][c] : s_c[1][lxy * AR + lz - BLK][c - 4];
-            dst[l] = v;
+            dst[l] = c < 4 ? s_q[node][c] : s_c[node][c - 4];
         }
         __syncthreads();
     }
@@ -2256,8 +2317,8 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
     const dim3 G(g_launch.bucket_wgs);
     if (A.cb == 8) {
-        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_cb8<T, 1>), G, dim3(128), 0, st, A);
-        else hipLaunchKernelGGL((k_p2g_cb8<T, 2>), G, dim3(128), 0, st, A);
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_cb8<T, 1>), G, dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g_cb8<T, 2>), G, dim3(64), 0, st, A);
         return;
     }
     // four fp64 arenas would exceed 64 KiB of LDS per work-group: fp64 always takes two
