@@ -1007,11 +1007,10 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
         p2g_scatter<T, 1, AZ>(sq, sc, lp1, q1);
         return;
     }
-    const bool two = left > 1;
-    P2GPart<T> q[2];
+    P2GPart<T> q[NP];
     int sh[3];
-    p2g_load(A, i, ng, q[0], sh);
-    p2g_load(A, i + (two ? 1 : 0), ng, q[1], sh);
+#pragma unroll
+    for (int j = 0; j < NP; ++j) p2g_load(A, i + (j < left ? j : 0), ng, q[j], sh);
     // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as arena
     // positions (not grid-clamped; positions beyond the grid fold onto the
     // edge node in the grid pass)
@@ -1021,13 +1020,21 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
     // particles of one cell share the frame unless they lie beyond the grid
-    // (cells clamped together); such a second particle goes alone
-    const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
-    if (!same) p2g_mute(q[1]);
-    p2g_scatter<T, 2, AZ>(sq, sc, lp, q);
-    if (two && !same) {
+    // (cells clamped together); such a particle goes alone after the others
+    bool alone[NP];
+#pragma unroll
+    for (int j = 1; j < NP; ++j) {
+        const bool has = j < left;
+        const bool same = has && q[j].bcb[0] == q[0].bcb[0] && q[j].bcb[1] == q[0].bcb[1] && q[j].bcb[2] == q[0].bcb[2];
+        alone[j] = has && !same;
+        if (!same) p2g_mute(q[j]);
+    }
+    p2g_scatter<T, NP, AZ>(sq, sc, lp, q);
+#pragma unroll
+    for (int j = 1; j < NP; ++j) {
+        if (!alone[j]) continue;
         P2GPart<T> q1[1];
-        p2g_load(A, i + 1, ng, q1[0], sh);
+        p2g_load(A, i + j, ng, q1[0], sh);
         int lp1[3][4];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -1148,7 +1155,7 @@ k_p2g_cb8(StepArgs<T> A) {
                 c0 = A.start[cell];
                 n = A.start[cell + 1] - c0;
             }
-            const unsigned pp = (unsigned)(NP == 2 ? (n + 1) >> 1 : n);
+            const unsigned pp = (unsigned)((n + NP - 1) / NP);
             const int idx = j * 64 + lane;
             s_key[idx] = ((pp < 0xFFFFu ? pp : 0xFFFFu) << 16) | (unsigned)(0xFFFF - idx);
             s_c0[idx] = c0;
@@ -1173,7 +1180,7 @@ k_p2g_cb8(StepArgs<T> A) {
         const int ia = s_item[lane], ib = s_item[127 - lane];
         int ca = s_c0[ia], na = s_n[ia];
         int cb = s_c0[ib], nbi = s_n[ib];
-        const int load = (NP == 2 ? (na + 1) >> 1 : na) + (NP == 2 ? (nbi + 1) >> 1 : nbi);
+        const int load = (na + NP - 1) / NP + (nbi + NP - 1) / NP;
         int rounds = load;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
         __syncthreads();                    // items read: the arena may be cleared
