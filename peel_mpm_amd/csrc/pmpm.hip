@@ -461,30 +461,78 @@ __global__ void __launch_bounds__(WG) k_bin_scatter(long long n, const int* __re
     }
 }
 
+// Sort the particle ids of n <= 16 entries ascending: an odd-even
+// transposition network in registers.
+__device__ __forceinline__ void sort16(int v[16]) {
+#pragma unroll
+    for (int round = 0; round < 16; ++round)
+#pragma unroll
+        for (int i = round & 1; i + 1 < 16; i += 2) {
+            const int lo = v[i] < v[i + 1] ? v[i] : v[i + 1];
+            const int hi = v[i] < v[i + 1] ? v[i + 1] : v[i];
+            v[i] = lo;
+            v[i + 1] = hi;
+        }
+}
+
 // Sort each cell's particle ids ascending (then perm is a stable argsort of
-// key): one lane per cell of each non-empty bucket, insertion sort in place.
+// key): one work-group of 64 threads per non-empty bucket.  The bucket's
+// ids are read into LDS with coalesced loads, lane l sorts the ids of cell
+// l there (a register network up to 16, insertion sort beyond), and the
+// bucket is written back with coalesced stores, with inv of each of its
+// particles.  A bucket of more than STAB_CAP particles sorts in place in
+// global memory instead.
+constexpr int STAB_CAP = 2048;
+constexpr int STAB_WGS = 8192;          // work-groups of k_bin_stabilize (grid-stride beyond)
 __global__ void __launch_bounds__(64) k_bin_stabilize(const int* __restrict__ start, const int* __restrict__ alist,
                                                       const int* __restrict__ misc, int* __restrict__ perm,
                                                       int* __restrict__ inv) {
+    __shared__ int s_id[STAB_CAP];
     const int nact = misc[0];
+    const int lane = threadIdx.x;
     for (int ai = blockIdx.x; ai < nact; ai += gridDim.x) {
-        const long long c = ((long long)alist[ai] << 6) + threadIdx.x;
+        const long long c = ((long long)alist[ai] << 6) + lane;
         const int s0 = start[c], s1 = start[c + 1], n = s1 - s0;
+        const int b0 = __shfl(s0, 0), b1 = __shfl(s1, 63);
+        const int nbk = b1 - b0;
+        if (nbk <= STAB_CAP) {
+            for (int i = lane; i < nbk; i += 64) s_id[i] = perm[b0 + i];
+            __syncthreads();
+            int* seg = s_id + (s0 - b0);
+            if (n > 1 && n <= 16) {
+                int v[16];
+#pragma unroll
+                for (int i = 0; i < 16; ++i) v[i] = i < n ? seg[i] : 0x7fffffff;
+                sort16(v);
+#pragma unroll
+                for (int i = 0; i < 16; ++i)
+                    if (i < n) seg[i] = v[i];
+            } else if (n > 16) {
+                for (int i = 1; i < n; ++i) {
+                    const int e = seg[i];
+                    int j = i - 1;
+                    while (j >= 0 && seg[j] > e) {
+                        seg[j + 1] = seg[j];
+                        --j;
+                    }
+                    seg[j + 1] = e;
+                }
+            }
+            __syncthreads();
+            for (int i = lane; i < nbk; i += 64) {
+                const int id = s_id[i];
+                perm[b0 + i] = id;
+                inv[id] = b0 + i;
+            }
+            __syncthreads();
+            continue;
+        }
         if (n <= 1) continue;
         if (n <= 16) {
-            // odd-even transposition sort of up to 16 ids in registers
             int v[16];
 #pragma unroll
             for (int i = 0; i < 16; ++i) v[i] = i < n ? perm[s0 + i] : 0x7fffffff;
-#pragma unroll
-            for (int round = 0; round < 16; ++round)
-#pragma unroll
-                for (int i = round & 1; i + 1 < 16; i += 2) {
-                    const int lo = v[i] < v[i + 1] ? v[i] : v[i + 1];
-                    const int hi = v[i] < v[i + 1] ? v[i + 1] : v[i];
-                    v[i] = lo;
-                    v[i + 1] = hi;
-                }
+            sort16(v);
 #pragma unroll
             for (int i = 0; i < 16; ++i)
                 if (i < n) {
@@ -2378,7 +2426,7 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)(nmark + (n > 0 ? nblocks(n) : 0))), dim3(WG), 0, st, n, key, slot, start,
                        perm, inv, nmark, mark, L.NB, blist, misc);
     if (stable && n > 0)
-        hipLaunchKernelGGL(k_bin_stabilize, dim3(PERSIST_WG), dim3(64), 0, st, start, alist, misc, perm, inv);
+        hipLaunchKernelGGL(k_bin_stabilize, dim3(STAB_WGS), dim3(64), 0, st, start, alist, misc, perm, inv);
     HIPCHK(hipGetLastError());
     return PMPM_OK;
 }

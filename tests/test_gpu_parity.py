@@ -131,6 +131,33 @@ def test_binning_stable_over_a_reused_workspace():
         assert np.array_equal(perm.cpu().numpy(), np.argsort(want, kind="stable")), name
 
 
+def test_binning_piles_take_every_sort_path():
+    """Cells and buckets far above the usual fill, shuffled: a cell of 40
+    particles (sorted in LDS by insertion), a bucket of 2,600 particles (more
+    than the LDS stage holds: sorted in place in global memory), cells of
+    2-16 (the register network) and singletons -- each still the stable
+    argsort."""
+    from peel_mpm_amd import kernels
+    ng = 32
+    g = np.random.default_rng(33)
+    parts = [
+        (np.array([10, 11, 12]) + g.uniform(0.1, 0.9, (40, 3))) / ng,          # one cell of 40
+        (np.array([20, 20, 20]) + g.uniform(0.0, 4.0, (2600, 3))) / ng,        # one bucket of 2,600
+        (np.array([21, 21, 21]) + g.uniform(0.2, 0.8, (300, 3))) / ng,         # and a pile in it
+        g.uniform(0.0, 1.0, (5000, 3)),
+    ]
+    x = np.concatenate(parts).astype(np.float32)
+    x = x[g.permutation(len(x))]
+    key, perm, start = kernels.bin_particles(torch.from_numpy(x).to(DEV), ng, stable=True)
+    torch.cuda.synchronize()
+    want = expected_cell_keys(torch.from_numpy(x), ng)
+    counts = np.bincount(want, minlength=kernels.n_cells(ng))
+    assert counts.max() > 16 and np.bincount(want // 64).max() > 2048
+    assert np.array_equal(key.cpu().numpy(), want)
+    assert np.array_equal(perm.cpu().numpy(), np.argsort(want, kind="stable"))
+    assert np.array_equal(start.cpu().numpy(), np.concatenate([[0], np.cumsum(counts)]))
+
+
 def test_binning_unstable_is_a_permutation_grouped_by_bucket():
     from peel_mpm_amd import kernels
     g = np.random.default_rng(9)
