@@ -66,9 +66,12 @@ HBM_PEAK_GBS = 8000.0
 KERNELS = ("binning", "stress_p2g", "grid_update", "g2p")
 PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage solve, one CG iteration
 
-# The plain step's pinned launch shape: the autotuner's pick for config1
-# (autotune.tune_solver on this scene), and the shape profiles/ were taken at.
-PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 1,
+# The plain step's pinned launch shape, and the shape profiles/ were taken
+# at: the fastest over config1's whole 500-step run in a coordinate sweep of
+# the autotuner's knobs.  autotune.tune_solver on the scene's first steps
+# picks p2g_waves 1 (the intact slab's uniform buckets); over the run, two
+# wavefronts per bucket measured 0.272 against 0.286 ms per step.
+PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 2,
                  "g2p_threads": 256, "cell_block": 4}
 
 FRAME = 10                          # substeps per run_frame call in the timed legs
