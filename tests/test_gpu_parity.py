@@ -4,8 +4,11 @@ Tolerances.  Integer work (cells, node indices, bucket keys, the stable sort
 permutation) is compared bit-exactly.  Floating fields use the norm-wise
 relative error ||a-b||/||b|| per field per step:
 
+* one step from a golden input against the reference's fp32 output: every
+  field 1e-5, the diagnostics (phi, phi_shear, tau_last, J_last) included
+  (measured worst over the cases: 3.5e-6, D_hist);
 * kinematic state x, v, F, C against the fp32 reference: 1e-5;
-* stress-derived fields (d, D_hist, lap, phi, phi_shear, tau_last) are
+* over several steps, stress-derived fields (d, D_hist, lap, phi, phi_shear, tau_last) are
   ill-conditioned in fp32 at small strain: the corotated stress is a
   difference F - R of O(1) numbers, so two fp32 implementations that round
   differently disagree by about ulp/strain.  They are held to the fp32 noise
@@ -28,7 +31,7 @@ from tests.golden_cases import (DIAG_KEYS, PARAM_KEYS, STATE_KEYS, STEP_CASES, a
 pytestmark = pytest.mark.gpu
 DEV = "cuda:0"
 TOL_STATE = {torch.float32: 1e-5, torch.float64: 1e-10}
-TOL_DIAG = {torch.float32: 1e-4, torch.float64: 1e-9}
+TOL_DIAG = TOL_STATE
 KINEMATIC = ("x", "v", "F", "C", "J_last")
 SENSITIVE = ("d", "D_hist", "lap", "phi", "phi_shear", "tau_last")
 NOISE_X = 3.0
