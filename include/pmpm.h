@@ -231,10 +231,10 @@ typedef struct {
     int32_t g2p_threads;    /* threads per work-group (one bucket) of G2P: 128, 256 or 512;
                                default 256 */
     int32_t cell_block;     /* cells along z of a P2G block: 4 (one 4x4x4-cell bucket per
-                               work-group, a 7x7x7-node arena each) or 8 (two buckets adjacent
-                               along z per work-group, one wavefront each, summed into one
-                               7x7x11-node arena, which the grid pass folds); default 4.  The
-                               PCG operator keeps 4. */
+                               work-group, a 7x7x7-node arena each) or 8 (the two buckets
+                               adjacent along z in one wavefront, their cells packed onto the
+                               lanes by particle count, into one 7x7x11-node arena, which the
+                               grid pass folds); default 4.  The PCG operator keeps 4. */
 } pmpm_launch_config;
 /* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
 int pmpm_set_launch_config(const pmpm_launch_config *cfg);

@@ -17,8 +17,9 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   kernel;
 * ``g2p_threads``: work-group size of G2P (one bucket per work-group);
 * ``cell_block``: the cell block of P2G and of the grid pass's fold -- 4
-  (4x4x4 cells, a 7x7x7-node arena per bucket) or 8 (4x4x8 cells: two
-  buckets adjacent along z share one 7x7x11-node arena, 21 % fewer arena
+  (4x4x4 cells, a 7x7x7-node arena per bucket) or 8 (4x4x8 cells: the two
+  buckets adjacent along z in one wavefront, their cells packed onto the
+  lanes by particle count, into one 7x7x11-node arena; 21 % fewer arena
   nodes written and folded).
 
 The bucket (4x4x4 cells, one cell per lane of a 64-wide wavefront) stays
