@@ -119,9 +119,10 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
     run_frame calls of frame_length(steps, frame) substeps (eager launches;
     use_graph replays a captured HIP graph per frame, which measured slower
     at this size), then the same run again with live per-group timing.  The
-    warm-up (whole frames, after any tuning) runs first and the initial
-    state is restored before each timed run, so both time steps 0..steps-1
-    of the scene: its cost grows as the slab tears and spreads.  Returns the
+    warm-up (``warmup`` steps, at least one, after any tuning) runs first
+    and the initial state is restored before each timed run, so both time
+    steps 0..steps-1 of the scene: its cost grows as the slab tears and
+    spreads.  Returns the
     solver, wall seconds, ms per launch and launches of each timing group,
     active nodes (mean over the run), and the launch shape used (with the
     frame length)."""
@@ -147,9 +148,9 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
 
     def run(k):
         for s in range(0, k, f):
-            sim.run_frame(f, s * dt)
+            sim.run_frame(min(f, k - s), s * dt)
 
-    run(-(-max(warmup, 1) // f) * f)
+    run(max(warmup, 1))
     restart()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
