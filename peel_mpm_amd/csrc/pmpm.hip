@@ -1332,12 +1332,14 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
         two[a] = i < 2 || i == U[a] - 1;
     }
     T v[8][NCH];
+    int cs[8];       // slot of candidate unit c, -1 when it does not hold the node
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
         const int cx = c >> 2, cy = (c >> 1) & 1, cz = c & 1;
         const int ux = lo[0] + cx, uy = lo[1] + cy, uz = lo[2] + cz;
         const int s = __shfl(lslot, (ux + 1) * 9 + (uy + 1) * 3 + (uz + 1));
         const bool ok = (cx == 0 || two[0]) && (cy == 0 || two[1]) && (cz == 0 || two[2]) && s >= 0;
+        cs[c] = ok ? s : -1;
         const int lx = g[0] - (BLK * (bx + ux) - 1), ly = g[1] - (BLK * (by + uy) - 1), lz = g[2] - (UZ * (u[2] + uz) - 1);
         const size_t off = ok ? (size_t)s * SLOT + 6 * ((lx * AR + ly) * ARZ + lz) : 0;
         load_node<T, NCH>(arena + off, v[c]);
@@ -1350,22 +1352,19 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
         for (int ch = 0; ch < NCH; ++ch) acc[ch] += v[c][ch];
     const bool edge = g[0] == 0 || g[1] == 0 || g[2] == 0 || g[0] == ng - 1 || g[1] == ng - 1 || g[2] == ng - 1;
     if (!off_grid || !edge) return;
-    const int nb = A.nb, nbz = CB == 1 ? nb : (nb + 1) >> 1;
+    // An arena position beyond the grid folds onto edge node g only from a
+    // unit that holds g itself: per axis, position -1 lies in unit 0 (which
+    // holds node 0), positions ng and ng+1 in the last unit (which holds
+    // ng-1).  So the extra positions come from the candidates above, taken
+    // in the same ascending unit order, each a box of at most 3 positions
+    // per axis around the node's own one.
     const int ext[3] = {AR, AR, ARZ};
-    for (int k = 0; k < 27; ++k) {
-        const int b3[3] = {u[0] - 1 + k / 9, u[1] - 1 + (k / 3) % 3, u[2] - 1 + k % 3};
-        if (b3[0] < 0 || b3[0] >= nb || b3[1] < 0 || b3[1] >= nb || b3[2] < 0 || b3[2] >= nbz) continue;
-        long long sk;
-        if constexpr (CB == 1) {
-            sk = A.slotof[scene * A.NBs + (b3[0] * nb + b3[1]) * nb + b3[2]];
-        } else {
-            const long long b0 = scene * A.NBs + (b3[0] * nb + b3[1]) * nb + 2 * b3[2];
-            const bool act = A.slotof[b0] >= 0 || (2 * b3[2] + 1 < nb && A.slotof[b0 + 1] >= 0);
-            sk = act ? scene * A.NBs8 + (b3[0] * nb + b3[1]) * nbz + b3[2] : -1;
-        }
+    for (int c = 0; c < 8; ++c) {
+        const int sk = cs[c];
         if (sk < 0) continue;
+        const int b3[3] = {u[0] + lo[0] + (c >> 2), u[1] + lo[1] + ((c >> 1) & 1), u[2] + lo[2] + (c & 1)};
         int lo3[3], hi3[3], own[3];
-        bool any_extra = false, ok = true;
+        bool any_extra = false;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             const int org = U[a] * b3[a] - 1;
@@ -1374,10 +1373,9 @@ __device__ __forceinline__ void fold_node(const StepArgs<T>& A, int scene, int l
             lo3[a] = (plo > org ? plo : org) - org;
             hi3[a] = (phi < org + ext[a] - 1 ? phi : org + ext[a] - 1) - org;
             own[a] = g[a] - org;
-            ok = ok && lo3[a] <= hi3[a];
-            any_extra = any_extra || lo3[a] != hi3[a] || lo3[a] != own[a];
+            any_extra = any_extra || lo3[a] != hi3[a];
         }
-        if (!ok || !any_extra) continue;
+        if (!any_extra) continue;
         const T* base = arena + (size_t)sk * SLOT;
         for (int lx = lo3[0]; lx <= hi3[0]; ++lx)
             for (int ly = lo3[1]; ly <= hi3[1]; ++ly)
