@@ -113,7 +113,7 @@ def frame_length(steps, frame):
     return f
 
 
-def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=False, **solver_kw):
+def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=False, count_cg=False, **solver_kw):
     """Time ``steps`` substeps of the scene from its initial state (the
     config's whole run at the default ``steps``), run as
     run_frame calls of frame_length(steps, frame) substeps (eager launches;
@@ -125,7 +125,9 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
     spreads.  Returns the
     solver, wall seconds, ms per launch and launches of each timing group,
     active nodes (mean over the run), and the launch shape used (with the
-    frame length)."""
+    frame length).  count_cg: a third, untimed pass of the same run as
+    single substeps reads the CG iterations each damage solve took (the
+    device's count) into groups["pcg_executed"] = (iterations, solves)."""
     import torch
     from peel_mpm_amd import AnisoMPM, _lib, autotune, scenes
     sim = scenes.make_solver(AnisoMPM, spec, device="cuda:0", use_graph=use_graph, **solver_kw)
@@ -177,6 +179,13 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
     L.pmpm_timing(0)
     sim.use_graph = use_graph
     groups = {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS + ("pcg_solve", "pcg_iteration"))}
+    if count_cg:
+        restart()
+        ran = 0
+        for s in range(steps):
+            sim.substep(s * dt)
+            ran += sim.last_damage_solve()[0]
+        groups["pcg_executed"] = (ran, steps)
     graphed = bool(use_graph and sim._graph_eligible())
     return sim, wall, groups, (n_q, n_c), {"config": launch, "source": "autotuned" if tune else "pinned",
                                            "tuning_s": round(t_tune, 2), "frame": f, "graph": graphed}
@@ -305,14 +314,17 @@ def time_pcg(spec, steps, warmup, label, prep=None):
     rooflines of the step, of the solve and of one CG iteration (HIP events
     around the solve's kernels and around each iteration's)."""
     import torch
-    sim, wall, groups, (n_q, n_c), launch = time_gpu(spec, steps, warmup, tune=False, prep=prep, damage_solver="pcg",
-                                                     pcg_tol=1e-6, pcg_max_iter=50)
+    sim, wall, groups, (n_q, n_c), launch = time_gpu(spec, steps, warmup, tune=False, prep=prep, count_cg=True,
+                                                     damage_solver="pcg", pcg_tol=1e-6, pcg_max_iter=50)
     it, rel = sim.last_damage_solve()
     n = sim.n
     solve_ms, solves = groups["pcg_solve"]
-    iter_ms, iters = groups["pcg_iteration"]
-    it_per_step = iters / max(solves, 1)
-    per_iter_ms = iter_ms / max(iters, 1)
+    iter_ms, issued = groups["pcg_iteration"]       # issued iterations (those past convergence exit early)
+    ran, ran_solves = groups["pcg_executed"]
+    it_per_step = ran / max(ran_solves, 1)
+    # the iterations' launch time over the iterations that ran: no figure when
+    # every solve converged at its Jacobi start
+    per_iter_ms = iter_ms / ran if ran > 0 else None
     applies = it_per_step + 1                       # the initial residual's operator application
     ab = pcg_apply_bytes(n, n_c)
     sb = step_bytes(n, n_q, n_c) + applies * ab
@@ -321,12 +333,14 @@ def time_pcg(spec, steps, warmup, label, prep=None):
                         "<= 50 iterations)",
             "value": n * steps / wall, "unit": "particle-steps/s", "ms_per_step": ms_step,
             "solve_ms_per_step": solve_ms / max(solves, 1), "iterations_per_step": it_per_step,
+            "issued_iterations_per_step": issued / max(solves, 1),
             "ms_per_iteration": per_iter_ms, "bytes_per_iteration": ab,
             "last_iterations": it, "last_residual": rel,
             "step_roofline": dict(step_roofline(sb, ms_step * 1e-3), peak=HBM_PEAK_GBS),
-            "iteration_roofline": dict(step_roofline(ab, per_iter_ms * 1e-3), peak=HBM_PEAK_GBS),
+            "iteration_roofline": (dict(step_roofline(ab, per_iter_ms * 1e-3), peak=HBM_PEAK_GBS)
+                                   if per_iter_ms else None),
             "active_nodes": {"quadratic": n_q, "cubic": n_c},
-            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in groups.items()},
+            "kernel_ms": {k: round(v[0] / max(v[1], 1), 5) for k, v in groups.items() if k != "pcg_executed"},
             "finite": bool(torch.isfinite(sim.d).all())}
 
 

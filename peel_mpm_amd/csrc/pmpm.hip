@@ -2130,8 +2130,9 @@ __global__ void __launch_bounds__(WG) k_pcg_update(StepArgs<T> A) {
 // One work-group: the partial dots in a fixed order, then the CG scalars.
 enum { PCG_RED_INIT = 0, PCG_RED_ALPHA = 1, PCG_RED_BETA = 2 };
 // hflag (host-mapped, NULL under stream capture): thread 0 publishes
-// (seq << 32) | (done ? 0xFFFFFFFF : iterations) with one 8-byte system-scope
-// store, for the host's wait between iteration chunks.
+// (seq << 32) | iterations done, with bit 31 set once the solve has stopped,
+// as one 8-byte system-scope store, for the host's wait between iteration
+// chunks.
 template <int MODE>
 __global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ part, int nparts, double* __restrict__ sc,
                                                    double tol, int max_iter, unsigned long long* hflag, unsigned seq) {
@@ -2180,7 +2181,8 @@ __global__ void __launch_bounds__(WG) k_pcg_reduce(const double* __restrict__ pa
         sc[PCG_DONE] = (rel <= tol || it >= (double)max_iter) ? 1.0 : 0.0;
     }
     if (hflag) {
-        const unsigned lo = sc[PCG_DONE] != 0.0 ? 0xFFFFFFFFu : (unsigned)sc[PCG_ITER];
+        // low word: the iterations done, with the top bit set once the solve has stopped
+        const unsigned lo = (unsigned)sc[PCG_ITER] | (sc[PCG_DONE] != 0.0 ? 0x80000000u : 0u);
         __hip_atomic_store(hflag, ((unsigned long long)seq << 32) | lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
@@ -2192,6 +2194,7 @@ struct PcgFlag {
     int dev;
     unsigned long long *host, *dptr;
     unsigned seq;
+    int hint;       // iterations the last solve polled to its end took: the next solve's first chunk
 };
 thread_local std::vector<PcgFlag> t_pcg_flags;
 
@@ -2205,20 +2208,24 @@ PcgFlag* pcg_flag() {
     if (hipHostMalloc(&h, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess) return nullptr;
     if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) return nullptr;
     memset(h, 0, 64);
-    t_pcg_flags.push_back({dev, (unsigned long long*)h, (unsigned long long*)d, 0});
+    t_pcg_flags.push_back({dev, (unsigned long long*)h, (unsigned long long*)d, 0, 1});
     return &t_pcg_flags.back();
 }
 
 // Wait for the published state of solve seq to show `issued` iterations
-// done or convergence: 1 converged, 0 not yet, -1 not seen within a second
-// (the caller then reads the device flag the synchronous way).
-int pcg_wait(const PcgFlag& f, int issued) {
+// done or the solve stopped: 1 stopped (ran = the iterations it took), 0 not
+// yet, -1 not seen within a second (the caller then reads the device flag
+// the synchronous way).
+int pcg_wait(const PcgFlag& f, int issued, int& ran) {
     const auto t0 = std::chrono::steady_clock::now();
     for (unsigned spin = 1;; ++spin) {
         const unsigned long long v = __atomic_load_n(f.host, __ATOMIC_ACQUIRE);
         if ((unsigned)(v >> 32) == f.seq) {
             const unsigned lo = (unsigned)v;
-            if (lo == 0xFFFFFFFFu) return 1;
+            if (lo & 0x80000000u) {
+                ran = (int)(lo & 0x7FFFFFFFu);
+                return 1;
+            }
             if (lo >= (unsigned)issued) return 0;
         }
         if ((spin & 255) == 0) {
@@ -2552,10 +2559,13 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_APPLY_DIRECT>), GG, GB, 0, st, A);
         hipLaunchKernelGGL(k_pcg_init<T>, GG, GB, 0, st, A);
-        // Iterations go out in chunks of 1, 2, 4, ... with a read of the done
-        // flag between chunks, so a solve that converges in a few iterations
-        // does not pay for pcg_max_iter launches (nor, converging in one, for
-        // a second iteration's launches).  The reductions publish the solve's
+        // Iterations go out in chunks with a read of the done flag between
+        // chunks, so a solve that converges in a few iterations does not pay
+        // for pcg_max_iter launches: first as many as the last solve polled
+        // to its end took (none when it converged at the start: the flag is
+        // read right after the start's reduction), then 1, 2, 4, ...
+        // Iterations issued past convergence exit at their first load but
+        // still cost their launches.  The reductions publish the solve's
         // state to pinned host memory, which the host polls: no copy and no
         // stream synchronisation between chunks.  Under stream capture no
         // host read is possible: all pcg_max_iter iterations are enqueued and
@@ -2574,7 +2584,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         }
         hipLaunchKernelGGL(k_pcg_reduce<PCG_RED_INIT>, dim3(1), GB, 0, st, (const double*)A.ppart, nparts, A.pscal,
                            A.pcg_tol, A.pcg_max, hflag, seq);
-        int issued = 0, chunk = 1;
+        int issued = 0, chunk = pf ? std::max(pf->hint, 0) : 1, next = 1;
         while (issued < p->pcg_max_iter) {
             const int k_end = capturing ? p->pcg_max_iter : std::min(p->pcg_max_iter, issued + chunk);
             for (; issued < k_end; ++issued) {
@@ -2589,15 +2599,20 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             }
             if (capturing || issued >= p->pcg_max_iter) break;
             HIPCHK(hipGetLastError());
-            int done = pf ? pcg_wait(*pf, issued) : -1;
+            int ran = -1;
+            int done = pf ? pcg_wait(*pf, issued, ran) : -1;
             if (done < 0) {
                 double flag = 0.0;
                 HIPCHK(hipMemcpyAsync(&flag, A.pscal + PCG_DONE, sizeof flag, hipMemcpyDeviceToHost, st));
                 HIPCHK(hipStreamSynchronize(st));
                 done = flag != 0.0;
             }
-            if (done) break;
-            chunk *= 2;
+            if (done) {
+                if (ran >= 0) pf->hint = ran;
+                break;
+            }
+            chunk = next;
+            next *= 2;
         }
         hipLaunchKernelGGL(k_pcg_interp<T>, PG, PB, 0, st, A);
     }
