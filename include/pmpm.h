@@ -112,6 +112,20 @@ typedef struct {
      * writes the table before each replay.  No counterpart in the reference
      * (its closures run on the host every step, anisompm.py:601, :744). */
     const struct pmpm_boundary_table *boundary_table;
+    /* Binning carried to the next step.  No counterpart in the reference
+     * (its step does not sort particles).  bin_next = 1: G2P also takes the
+     * next step's cell keys and per-cell counts from the positions it writes
+     * (out->x), into ws (not with keep_for_adjoint: PMPM_EARG).
+     * prebinned = 1: the caller vouches that in->x holds, unchanged, the
+     * positions the previous step on ws wrote with bin_next (no edit in
+     * between); the step then skips its count pass.  The library takes that
+     * up only when the previous call on ws set bin_next with out->x == in->x
+     * and the same particle count, in the same stream capture (or both
+     * outside one), and counts afresh otherwise; an edit of x in place it
+     * cannot see, so a false claim after one gives a wrong step.  The
+     * binning is the same either way (with stable_sort, bit for bit). */
+    int32_t bin_next;
+    int32_t prebinned;
 } pmpm_step_params;
 
 /* The device-resident form of the step's boundary values (see
@@ -167,6 +181,9 @@ int pmpm_workspace_init(void *ws, size_t ws_bytes, int64_t max_n, int32_t n_grid
  * particle_bc hooks (replaced by bc_id/vel_bc) and with device colliders. */
 int pmpm_substep(const pmpm_step_params *p, const pmpm_particles_in *in,
                  const pmpm_particles_out *out, void *ws, size_t ws_bytes, void *stream);
+/* Steps on ws that started from a carried count (prebinned taken up)
+ * since ws was set up.  No counterpart in the reference. */
+int pmpm_carried_steps(const void *ws, int64_t *steps);
 /* The same step split at the collider call (anisompm.py:736-746) so a host
  * may apply arbitrary collider callables to the grid velocity in between:
  * begin = bin + stress/damage + P2G + grid update (+ device colliders);

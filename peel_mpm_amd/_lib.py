@@ -50,7 +50,8 @@ class StepParams(C.Structure):
                 ("stable_sort", C.c_int32), ("export_active", C.c_int32), ("exact_sigma_eig", C.c_int32),
                 ("n_scenes", C.c_int32), ("scene_offsets", C.c_void_p), ("scene_damage", C.c_void_p),
                 ("damage_solver", C.c_int32), ("pcg_tol", C.c_double), ("pcg_max_iter", C.c_int32),
-                ("keep_for_adjoint", C.c_int32), ("boundary_table", C.c_void_p)]
+                ("keep_for_adjoint", C.c_int32), ("boundary_table", C.c_void_p),
+                ("bin_next", C.c_int32), ("prebinned", C.c_int32)]
 
 
 class BoundaryTable(C.Structure):
@@ -112,6 +113,7 @@ def lib():
     L.pmpm_stencil.argtypes = [vp, i32, i64, i32, dbl, vp, vp, vp, vp, vp, vp, vp, vp]
     L.pmpm_sym_eig.argtypes = [vp, i32, i64, vp, vp, vp]
     L.pmpm_damage_solve_info.argtypes = [vp, i64, i32, i32, i32, C.POINTER(i32), C.POINTER(dbl), vp]
+    L.pmpm_carried_steps.argtypes = [vp, C.POINTER(i64)]
     L.pmpm_set_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.pmpm_get_launch_config.argtypes = [C.POINTER(LaunchConfig)]
     L.pmpm_last_error.restype = C.c_char_p

@@ -136,8 +136,8 @@ pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4};
 // The sections sized by n (arena, then the per-particle ones) follow; the
 // step rewrites them in full every call.
 struct Layout {
-    size_t arena, gv, active, count, start, btot, bstart, bflag, tiles, mark, alist, slotof, blist, key, slot, perm, inv, rec,
-        dsrt, misc, stats, pcgd, pcgr, gmw, total;
+    size_t arena, gv, active, count, count2, start, btot, bstart, bflag, bflag2, tiles, mark, alist, slotof, blist, key, slot,
+        perm, inv, rec, dsrt, misc, stats, pcgd, pcgr, gmw, total;
     long long NB, NBs, NC, ntiles, G, Gs, max_slots;
     // ns scenes, each with its own n_grid^3 grid and buckets (scene-major)
     Layout(int64_t n, int ng, int dtype, int ns = 1) {
@@ -154,9 +154,12 @@ struct Layout {
         size_t o = 0;
         // ---- sections independent of n
         misc = o;   o = al(o + 64);          // [0] n_active buckets, [1] n marked blocks, [2] grid partials,
-                                             // [3] some stencil leaves the grid (this step), [4] its raw flag
+                                             // [3] some stencil leaves the grid (this step), [4] its raw flag,
+                                             // [5] the raw flag of the carried count (G2P's)
         count = o;  o = al(o + (size_t)NC * 4);
         bflag = o;  o = al(o + (size_t)NB * 4);
+        count2 = o; o = al(o + (size_t)NC * 4);     // the carried count: G2P's, for the next step
+        bflag2 = o; o = al(o + (size_t)NB * 4);
         mark = o;   o = al(o + (size_t)NB * 4);
         start = o;  o = al(o + (size_t)(NC + 1) * 4);
         btot = o;   o = al(o + (size_t)NB * 4);
@@ -199,9 +202,67 @@ struct WsShape {
 std::mutex g_ws_mu;
 std::unordered_map<uintptr_t, WsShape> g_ws;
 
+void ws_register(const void* ws, const WsShape& s);
+
+// The carried binning of each workspace, by address: set by a step whose
+// G2P counted the next step's cells (bin_next), with the positions it wrote
+// and the stream capture it was enqueued in (0: none).  A later step on the
+// workspace takes it up (prebinned) only when its x is those positions, for
+// the same particle count, in the same capture (or both outside one); any
+// step or binning on the workspace consumes the record, since its scan
+// clears the carried count.
+struct Carry {
+    const void* x;
+    int64_t n;
+    unsigned long long cap;
+    bool pending;
+    long long taken;                        // steps that started from a carried count
+};
+std::unordered_map<uintptr_t, Carry> g_carry;
+
+unsigned long long capture_id(hipStream_t st) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    unsigned long long id = 0;
+    if (hipStreamGetCaptureInfo(st, &cs, &id) != hipSuccess || cs != hipStreamCaptureStatusActive) return 0;
+    return id ? id : 1;
+}
+
+// Whether this step on ws may start from the carried count (want: the
+// caller vouches that x is unchanged since the step that wrote it); the
+// record is consumed either way.
+bool carry_take(const void* ws, bool want, const void* x, int64_t n, hipStream_t st) {
+    if (!ws) return false;
+    const unsigned long long cap = capture_id(st);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_carry.find((uintptr_t)ws);
+    if (it == g_carry.end()) return false;
+    const Carry c = it->second;
+    it->second.pending = false;
+    const bool ok = want && c.pending && c.x == x && c.n == n && c.cap == cap && n > 0;
+    if (ok) ++it->second.taken;
+    return ok;
+}
+
+void carry_set(const void* ws, const void* x, int64_t n, hipStream_t st) {
+    const unsigned long long cap = capture_id(st);
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Carry& c = g_carry[(uintptr_t)ws];
+    c.x = x;
+    c.n = n;
+    c.cap = cap;
+    c.pending = true;
+}
+
+long long carry_taken(const void* ws) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    auto it = g_carry.find((uintptr_t)ws);
+    return it == g_carry.end() ? 0 : it->second.taken;
+}
+
 void ws_register(const void* ws, const WsShape& s) {
     std::lock_guard<std::mutex> lk(g_ws_mu);
     g_ws[(uintptr_t)ws] = s;
+    g_carry.erase((uintptr_t)ws);           // the set-up clears the workspace
 }
 
 int ws_check(const void* ws, int64_t n, int ng, int ns, int dtype) {
@@ -229,12 +290,12 @@ int ws_check(const void* ws, int64_t n, int ng, int ns, int dtype) {
 template <typename T>
 // off_grid: the particle's cubic stencil [c-1, c+2] (which holds its
 // quadratic one) reaches past the grid on some axis.
-__device__ __forceinline__ int cell_key(const T* __restrict__ x, long long p, T inv_dx, int ng, int nb, bool& off_grid) {
+__device__ __forceinline__ int cell_key(const T xp[3], T inv_dx, int ng, int nb, bool& off_grid) {
     int b[3], l[3];
     off_grid = false;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        const int c = floor_idx(scaled(x[3 * p + a], inv_dx), ng);
+        const int c = floor_idx(scaled(xp[a], inv_dx), ng);
         off_grid = off_grid || c < 1 || c + 2 > ng - 1;
         b[a] = clampi(c >> 2, 0, nb - 1);
         l[a] = clampi(c - BLK * b[a], 0, BLK - 1);
@@ -284,7 +345,8 @@ __global__ void __launch_bounds__(WG) k_bin_count(const T* __restrict__ x, long 
     int k = -1;
     bool off_grid = false;
     if (live) {
-        k = cell_key(x, p, inv_dx, ng, nb, off_grid);
+        const T xp[3] = {x[3 * p], x[3 * p + 1], x[3 * p + 2]};
+        k = cell_key(xp, inv_dx, ng, nb, off_grid);
         if (ns > 1) k += scene_of(scene_off, ns, p) * (nb * nb * nb * 64);
     }
     if (__ballot(off_grid) != 0 && lane == 0) misc[4] = 1;
@@ -339,14 +401,18 @@ __global__ void __launch_bounds__(WG) k_bucket_tiles(const int* __restrict__ bfl
 // every bucket), the ordered list of non-empty buckets (alist) and their
 // slots (slotof, -1 when empty), the cell starts of non-empty buckets (and
 // the end of each one's last cell), marks on the grid blocks their arenas
-// reach.  Clears count and bflag of the buckets it read.  Work-group 0 also
-// publishes k_bin_count's off-grid flag (misc[3] = misc[4], clearing
-// misc[4]) and resets misc[1]; the last writes misc[0] = non-empty buckets
-// and start[NC] = particles.
+// reach.  Clears count and bflag of the buckets it read, and the flagged
+// buckets of the other count set (bflag_o, count_o; null when none), so a
+// carried count that no step took up is gone.  Work-group 0 also publishes
+// the off-grid flags of the count and of the carried count (misc[3] =
+// misc[4] | misc[5], clearing both: a stale carried flag only makes the
+// grid pass look beyond the grid needlessly) and resets misc[1]; the last
+// writes misc[0] = non-empty buckets and start[NC] = particles.
 __global__ void __launch_bounds__(WG) k_bucket_apply(int* __restrict__ bflag, int* __restrict__ count, const int* __restrict__ btot,
                                                      long long NB, int nb, const int* __restrict__ tiles, int* __restrict__ bstart,
                                                      int* __restrict__ alist, int* __restrict__ slotof, int* __restrict__ mark,
-                                                     int* __restrict__ start, int* __restrict__ misc) {
+                                                     int* __restrict__ start, int* __restrict__ misc, int* __restrict__ bflag_o,
+                                                     int* __restrict__ count_o) {
     __shared__ int s_a[WG / 64], s_b[WG / 64], s_pa, s_pb;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // prefix of the earlier tiles (wave 0)
@@ -376,6 +442,12 @@ __global__ void __launch_bounds__(WG) k_bucket_apply(int* __restrict__ bflag, in
         bstart[b] = pa;
         slotof[b] = t > 0 ? pb : -1;
         if (bflag[b]) bflag[b] = 0;
+        if (bflag_o && bflag_o[b]) {
+            int4* c4 = reinterpret_cast<int4*>(count_o + (b << 6));
+#pragma unroll
+            for (int i = 0; i < 16; ++i) c4[i] = make_int4(0, 0, 0, 0);
+            bflag_o[b] = 0;
+        }
         if (t > 0) {
             alist[pb] = (int)b;
             int4* c4 = reinterpret_cast<int4*>(count + (b << 6));
@@ -407,8 +479,9 @@ __global__ void __launch_bounds__(WG) k_bucket_apply(int* __restrict__ bflag, in
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        misc[3] = misc[4];
+        misc[3] = misc[4] | misc[5];
         misc[4] = 0;
+        misc[5] = 0;
         misc[1] = 0;
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == WG - 1) {
@@ -570,6 +643,11 @@ template <typename T> struct StepArgs {
     uint8_t* active;
     const int *start, *alist, *slotof, *blist, *perm, *misc, *key, *inv;
     T *rec, *dsrt;              // P2G records and post-update d at sorted positions
+    // the carried binning: with bin_next, G2P takes the next step's cell keys
+    // and slots (ckey, cslot) and counts into the carried set (ccount, cbflag,
+    // cmisc[5]) from the positions it writes
+    int *ckey, *cslot, *ccount, *cbflag, *cmisc;
+    int bin_next;
     unsigned long long* stats;
     // scalars
     long long n, NBs, Gs;
@@ -1626,8 +1704,9 @@ __global__ void __launch_bounds__(NT) k_g2p(StepArgs<T> A) {
         }
         __syncthreads();
         // one particle: velocity, affine C and the damage Laplacian from the
-        // staged nodes, then advection, the F update and the clamp
-        auto particle = [&](long long p, const T (&xp)[3], const M3<T>& F) {
+        // staged nodes, then advection, the F update and the clamp; nk, off:
+        // the cell key of the new position and its off-grid flag (bin_next)
+        auto particle = [&](long long p, const T (&xp)[3], const M3<T>& F, int& nk, bool& off) {
             T Xp[3] = {scaled(xp[0], A.inv_dx), scaled(xp[1], A.inv_dx), scaled(xp[2], A.inv_dx)};
             int bq[3];
             T fx[3], w[3][3];
@@ -1664,21 +1743,58 @@ __global__ void __launch_bounds__(NT) k_g2p(StepArgs<T> A) {
                 for (int b = 0; b < 3; ++b) Ip.m[a][b] = (a == b ? T(1) : T(0)) + A.dt * Cn.m[a][b];
             store_m3(A.oF, p, mul(Ip, F));
             store_m3(A.oC, p, Cn);
+            T xw[3];
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 A.ov[3 * p + a] = vn[a];
                 const T xn = xp[a] + A.dt * vn[a];
-                A.ox[3 * p + a] = minp(maxp(xn, A.pad_lo), A.pad_hi);
+                xw[a] = minp(maxp(xn, A.pad_lo), A.pad_hi);
+                A.ox[3 * p + a] = xw[a];
             }
             A.olap[p] = lap;
+            if (A.bin_next) nk = cell_key(xw, A.inv_dx, ng, nb, off) + scene * (int)(A.NBs << 6);
         };
+        // the next step's binning from the written positions, as k_bin_count
+        // takes it from x: the lanes hold consecutive sorted particles, so
+        // runs of equal keys are long (one count atomic per run)
+        auto carry = [&](long long p, int nk, bool live, bool off) {
+            const int lane = threadIdx.x & 63;
+            if (__ballot(off) != 0 && lane == 0) A.cmisc[5] = 1;
+            bool head;
+            int first, len;
+            wave_run(nk, live, lane, head, first, len);
+            int base = 0;
+            if (head) base = atomicAdd(&A.ccount[nk], len);
+            base = __shfl(base, first);
+            if (live) {
+                A.ckey[p] = nk;
+                A.cslot[p] = base + (lane - first);
+            }
+            const int kb = nk >> 6, kbprev = __shfl_up(kb, 1);
+            if (live && (lane == 0 || kbprev != kb)) A.cbflag[kb] = 1;
+        };
+        // (the lanes of a wavefront run these loops together, live or not,
+        // so carry's wavefront operations see every lane)
 #pragma unroll
-        for (int k = 0; k < G2P_PREFETCH; ++k)
-            if ((int)threadIdx.x + k * NT < cnt) particle(pf[k], xf[k], Ff[k]);
-        for (int i = threadIdx.x + G2P_PREFETCH * NT; i < cnt; i += NT) {
-            const long long p = A.perm[s0 + i];
-            const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
-            particle(p, xp, load_m3(Fin, p));
+        for (int k = 0; k < G2P_PREFETCH; ++k) {
+            const bool live = (int)threadIdx.x + k * NT < cnt;
+            int nk = -1;
+            bool off = false;
+            if (live) particle(pf[k], xf[k], Ff[k], nk, off);
+            if (A.bin_next) carry(pf[k], nk, live, off);
+        }
+        for (int i0 = (int)(threadIdx.x & ~63u) + G2P_PREFETCH * NT; i0 < cnt; i0 += NT) {
+            const int i = i0 + (int)(threadIdx.x & 63);
+            const bool live = i < cnt;
+            long long p = 0;
+            int nk = -1;
+            bool off = false;
+            if (live) {
+                p = A.perm[s0 + i];
+                const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
+                particle(p, xp, load_m3(Fin, p), nk, off);
+            }
+            if (A.bin_next) carry(p, nk, live, off);
         }
         __syncthreads();
     }
@@ -2400,11 +2516,15 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
     }
 }
 
+// The binning.  pre: the carried count of the previous step's G2P (count2,
+// bflag2, and the keys and slots it wrote) is this x's, so the count pass is
+// skipped; otherwise k_bin_count counts afresh and the scan also clears the
+// carried set.
 template <typename T>
 int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const long long* scene_off, int ns, const Layout& L,
-            char* ws, hipStream_t st) {
+            char* ws, hipStream_t st, bool pre = false) {
     const int nb = nblk(ng);
-    int* count = (int*)(ws + L.count);
+    int* count = (int*)(ws + (pre ? L.count2 : L.count));
     int* start = (int*)(ws + L.start);
     int* btot = (int*)(ws + L.btot);
     int* bstart = (int*)(ws + L.bstart);
@@ -2416,17 +2536,17 @@ int run_bin(const T* x, long long n, int ng, double inv_dx, int stable, const lo
     int* perm = (int*)(ws + L.perm);
     int* inv = (int*)(ws + L.inv);
     int* misc = (int*)(ws + L.misc);
-    int* bflag = (int*)(ws + L.bflag);
+    int* bflag = (int*)(ws + (pre ? L.bflag2 : L.bflag));
     int* tiles = (int*)(ws + L.tiles);
     int* mark = (int*)(ws + L.mark);
     TimedScope ts(0, st);
-    if (n > 0)
+    if (n > 0 && !pre)
         hipLaunchKernelGGL(k_bin_count<T>, dim3(nblocks(n)), dim3(WG), 0, st, x, n, T(inv_dx), ng, nb, scene_off, ns, count, bflag,
                            key, slot, misc);
     const dim3 TG((unsigned)L.ntiles);
     hipLaunchKernelGGL(k_bucket_tiles, TG, dim3(WG), 0, st, bflag, count, L.NB, btot, tiles);
     hipLaunchKernelGGL(k_bucket_apply, TG, dim3(WG), 0, st, bflag, count, btot, L.NB, nb, tiles, bstart, alist, slotof, mark,
-                       start, misc);
+                       start, misc, pre ? nullptr : (int*)(ws + L.bflag2), pre ? nullptr : (int*)(ws + L.count2));
     const long long nmark = nblocks(L.NB);
     hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)(nmark + (n > 0 ? nblocks(n) : 0))), dim3(WG), 0, st, n, key, slot, start,
                        perm, inv, nmark, mark, L.NB, blist, misc);
@@ -2456,6 +2576,9 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.rec = (T*)(ws + L.rec); A.dsrt = (T*)(ws + L.dsrt);
     A.slotof = (const int*)(ws + L.slotof); A.blist = (const int*)(ws + L.blist);
     A.misc = (const int*)(ws + L.misc);
+    A.ckey = (int*)(ws + L.key); A.cslot = (int*)(ws + L.slot);
+    A.ccount = (int*)(ws + L.count2); A.cbflag = (int*)(ws + L.bflag2); A.cmisc = (int*)(ws + L.misc);
+    A.bin_next = 0;
     A.stats = (unsigned long long*)(ws + L.stats);
     A.gmw = p->keep_for_adjoint ? (T*)(ws + L.gmw) : nullptr;
     A.pdiag = (T*)(ws + L.pcgd);
@@ -2507,6 +2630,7 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             return fail(PMPM_EARG, "%spcg_max_iter out of range: %lld", "", p->pcg_max_iter);
         if (!(p->pcg_tol >= 0.0)) return fail(PMPM_EARG, "%spcg_tol must be >= 0", "");
     }
+    if (p->bin_next && p->keep_for_adjoint) return fail(PMPM_EARG, "%sbin_next does not combine with keep_for_adjoint", "");
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     if (ws) {
@@ -2534,12 +2658,15 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
     return PMPM_OK;
 }
 
+// carry: the workspace may carry binning between steps (not the
+// no-allocation scratch, which is cleared every call).
 template <typename T>
 int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
-               int exact_sigma, bool host_poll = true) {
+               int exact_sigma, bool host_poll = true, bool carry = true) {
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
+    const bool pre = carry && carry_take(ws, p->prebinned != 0, in->x, in->n, st);
     int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
-                        p->n_scenes, L, ws, st);
+                        p->n_scenes, L, ws, st, pre);
     if (rc) return rc;
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
     if (p->export_active)
@@ -2636,9 +2763,14 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
 }
 
 template <typename T>
-int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st) {
+int step_end(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
+             bool carry = true) {
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
+    if (carry && p->bin_next && in->n > 0) {
+        A.bin_next = 1;
+        carry_set(ws, out->x, in->n, st);
+    }
     if (in->n > 0) {
         TimedScope ts(3, st);
         const dim3 G(g_launch.bucket_wgs);
@@ -2819,6 +2951,12 @@ int pmpm_substep(const pmpm_step_params* p, const pmpm_particles_in* in, const p
     return pmpm_substep_end(p, in, out, ws, ws_bytes, stream);
 }
 
+int pmpm_carried_steps(const void* ws, int64_t* steps) {
+    if (!ws || !steps) return fail(PMPM_EARG, "%snull argument", "");
+    *steps = (int64_t)carry_taken(ws);
+    return PMPM_OK;
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------- no-allocation step
@@ -2876,10 +3014,11 @@ int pmpm_substep_noalloc(const pmpm_step_params* p, const pmpm_particles_in* in,
     hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks((nz / 16) < 1 ? 1 : nz / 16)), dim3(WG), 0, st, (unsigned char*)ws, nz);
     HIPCHK(hipGetLastError());
     t_untimed = true;
-    int rc = p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false)
-                                  : step_begin<float>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false);
+    int rc = p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false)
+                                  : step_begin<float>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false);
     if (!rc)
-        rc = p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, st) : step_end<float>(p, in, out, (char*)ws, st);
+        rc = p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, st, false)
+                                  : step_end<float>(p, in, out, (char*)ws, st, false);
     t_untimed = false;
     return rc;
 }
@@ -2931,6 +3070,7 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
         for (const void* q : need)
             if (!q) return fail(PMPM_EARG, "%smissing adjoint array", "");
     }
+    carry_take(ws, false, nullptr, 0, (hipStream_t)stream);
     return p->dtype == PMPM_F64
                ? step_adjoint<double>(p, in, fwd_out, gin, gout, (char*)ws, (char*)aws, (hipStream_t)stream)
                : step_adjoint<float>(p, in, fwd_out, gin, gout, (char*)ws, (char*)aws, (hipStream_t)stream);
@@ -2945,6 +3085,7 @@ int pmpm_bin(const void* x, int32_t dtype, int64_t n, int32_t n_grid, double inv
     if (ws_bytes < L.total) return fail(PMPM_ESIZE, "%sworkspace too small, need %lld bytes", "", (long long)L.total);
     if (int rc = ws_check(ws, n, n_grid, 1, dtype)) return rc;
     hipStream_t st = (hipStream_t)stream;
+    carry_take(ws, false, nullptr, 0, st);
     int rc = dtype == PMPM_F64 ? run_bin<double>((const double*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st)
                                : run_bin<float>((const float*)x, n, n_grid, inv_dx, stable, nullptr, 1, L, (char*)ws, st);
     if (rc) return rc;

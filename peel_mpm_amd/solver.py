@@ -95,6 +95,10 @@ class AnisoMPM:
             raise ValueError("frame_diagnostics is 'end' (written on the frame's last substep) or 'every'")
         self.frame_diagnostics = frame_diagnostics
         self._graph = None
+        # (x's address, version, particle count) after a step whose G2P also
+        # counted the next step's binning; the next step may skip its count
+        # pass while x is still that tensor, unmodified
+        self._carry = None
         self._pin = _lib.ParticlesIn()
         self._out_n = -1
         _lib.lib()
@@ -299,6 +303,12 @@ class AnisoMPM:
 
         p = self._step_params()
         p.do_damage = int(do_damage)
+        # the binning: the previous step counted this one's from the x it
+        # wrote; valid while x is that tensor, unmodified (checked here, after
+        # the host BC hooks, and again by the library)
+        p.bin_next = 1
+        p.prebinned = int(self._carry is not None and
+                          self._carry == (self.x.data_ptr(), self.x._version, self.n))
         device_cols = len(self.colliders) <= _lib.MAX_COLLIDERS and \
             all(isinstance(c, (SphereCollider, HalfspaceCollider)) for c in self.colliders)
         p.n_colliders = len(self.colliders) if device_cols else 0
@@ -344,6 +354,7 @@ class AnisoMPM:
             _lib.check(L.pmpm_substep_end(*args), "pmpm_substep_end")
         for k, t in new.items():
             setattr(self, k, t)
+        self._carry = (self.x.data_ptr(), self.x._version, self.n)
         self._step += 1
 
     def _grid_views(self):
@@ -472,6 +483,7 @@ class AnisoMPM:
                     f()
             G["graphs"][key] = g
         g.replay()
+        self._carry = None
         cur = (cur + n_sub) % 2
         G["cur"] = cur
         for k in _STATE:
@@ -499,6 +511,8 @@ class AnisoMPM:
             p.n_colliders = len(self.colliders)
             p.n_vel_bc = len(self.particle_bc)
             p.export_active = 0
+            # the binning carried from substep to substep inside the frame
+            p.bin_next, p.prebinned = int(s < n_sub - 1), int(s > 0)
             pin = _lib.ParticlesIn()
             pin.n = self.n
             for k in _STATE:
@@ -527,6 +541,15 @@ class AnisoMPM:
         _lib.check(_lib.lib().pmpm_last_step_stats(self._ws.data_ptr(), self.n, self.n_grid, 1, self._code, C.byref(q),
                                                    C.byref(c), _lib.stream_ptr(self.device)), "pmpm_last_step_stats")
         return q.value, c.value
+
+    def carried_steps(self):
+        """Steps on this solver's workspace that skipped their count pass,
+        starting from the binning the previous step's G2P counted."""
+        n = C.c_int64()
+        if self._ws is None:
+            return 0
+        _lib.check(_lib.lib().pmpm_carried_steps(self._ws.data_ptr(), C.byref(n)), "pmpm_carried_steps")
+        return n.value
 
     def last_damage_solve(self):
         """(iterations, relative residual) of the most recent PCG damage solve
