@@ -199,7 +199,7 @@ int pmpm_substep_end(const pmpm_step_params *p, const pmpm_particles_in *in,
  * PMPM_NOALLOC_SCRATCH_BYTES (64 MiB) of device memory the library itself
  * holds, a static device array that every GPU the library runs on keeps
  * until the process ends -- enough for a 64^3-grid scene of up to about
- * 250,000 fp32 particles (pmpm_workspace_bytes(n, n_grid, n_scenes, dtype)
+ * 217,000 fp32 particles (pmpm_workspace_bytes(n, n_grid, n_scenes, dtype)
  * is the need; a larger step returns PMPM_ESIZE before anything is
  * enqueued).  Every call on a GPU shares that GPU's scratch and leaves its
  * contents behind; the call clears what must start at zero itself.  Calls
