@@ -148,7 +148,24 @@ typedef struct {
     const uint8_t *allow, *interface_mask;
     const void *n_interface;
     const uint8_t *bc_id;
+    /* Uniform material fields.  Bit k of uniform_mask set: every particle holds
+     * the same value of field k, given in uniform_value[k] (A0: the 3x3
+     * matrix in uniform_A0, row-major), and the step reads that value instead
+     * of the array, whose pointer may then be NULL.  Bits, in order: vol0,
+     * mass, mu, lam, sigma_c, eta, zeta, l0, residual, A0 (PMPM_U_*).  The
+     * values must be the fields' own (converted from the step's dtype), so
+     * the step computes the same bits either way.  No counterpart in the
+     * reference (its fields are per-particle tensors, though add_object fills
+     * each from one scalar); pmpm_substep_adjoint takes no uniform fields. */
+    uint32_t uniform_mask;
+    double uniform_value[9];
+    double uniform_A0[9];
 } pmpm_particles_in;
+
+enum {
+    PMPM_U_VOL0 = 0, PMPM_U_MASS = 1, PMPM_U_MU = 2, PMPM_U_LAM = 3, PMPM_U_SIGMA_C = 4,
+    PMPM_U_ETA = 5, PMPM_U_ZETA = 6, PMPM_U_L0 = 7, PMPM_U_RESIDUAL = 8, PMPM_U_A0 = 9
+};
 
 /* Outputs.  x, v, F, C, lap are always written, and so are tau_last and
  * J_last unless both are NULL.  d, D_hist, phi, phi_shear are written only

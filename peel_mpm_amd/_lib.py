@@ -81,7 +81,12 @@ class FrameObs(C.Structure):
 
 
 class ParticlesIn(C.Structure):
-    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in _IN_PTRS]
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in _IN_PTRS] + \
+        [("uniform_mask", C.c_uint32), ("uniform_value", C.c_double * 9), ("uniform_A0", C.c_double * 9)]
+
+
+# the material fields a step may take as one value (bit k of uniform_mask)
+UNIFORM_FIELDS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "residual", "A0")
 
 
 class ParticlesOut(C.Structure):

@@ -28,7 +28,7 @@ import ctypes as C
 
 import torch
 
-from . import _lib
+from . import _lib, uniform
 from .boundaries import DeviceBoundaryTable, HalfspaceCollider, SphereCollider, VelocityBC
 
 _STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
@@ -58,6 +58,7 @@ class BatchedAnisoMPM:
         self.colliders = []
         self._parts = []          # per scene: dict of tensors and its VelocityBC list
         self._built = False
+        self._uf = uniform.UniformFields()
         self._step = 0
 
     # ------------------------------------------------------------------ scenes
@@ -187,6 +188,7 @@ class BatchedAnisoMPM:
         pin.allow = self.allow.data_ptr()
         pin.interface_mask = self.interface_mask.data_ptr()
         pin.bc_id = self._bc_id.data_ptr()
+        uniform.fill(pin, self._uf.state(self))
         pout = _lib.ParticlesOut()
         for k in _STATE:
             setattr(pout, k, dst[k].data_ptr())
@@ -203,9 +205,10 @@ class BatchedAnisoMPM:
 
     def _capture_key(self):
         """What a captured graph holds by value or by address besides the
-        ping-pong buffers: the parameter tensors and the collider count."""
+        ping-pong buffers: the parameter tensors, the collider count and the
+        uniform material values (kernel arguments)."""
         return tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS + ("allow", "interface_mask")) + \
-            (len(self.colliders),)
+            (len(self.colliders), uniform.key(self._uf.state(self)))
 
     def substep(self, time: float = 0.0):
         if not self._built:

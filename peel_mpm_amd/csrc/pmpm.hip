@@ -633,6 +633,10 @@ template <typename T> struct StepArgs {
     const T *x, *v, *F, *C, *d, *lap, *D_hist, *vol0, *mass, *mu, *lam, *sigma_c, *eta, *zeta, *l0, *residual, *A0,
         *n_if;
     const uint8_t *allow, *imask, *bc_id;
+    // uniform material fields (bit k of umask: field k is uval[k], A0 is uA0)
+    unsigned umask;
+    T uval[9];
+    M3<T> uA0;
     // particles out
     T *ox, *ov, *oF, *oC, *olap, *od, *oD, *ophi, *ophs, *otau, *oJ;
     // workspace
@@ -698,6 +702,12 @@ struct BucketIter {
     }
 };
 
+// Material field K of particle p: the uniform value when the caller passed
+// one, else the array's entry.
+template <int K, typename T> __device__ __forceinline__ T fld(const StepArgs<T>& A, const T* __restrict__ a, long long p) {
+    return ((A.umask >> K) & 1u) ? A.uval[K] : a[p];
+}
+
 template <typename T> __device__ __forceinline__ M3<T> load_m3(const T* __restrict__ a, long long p) {
     M3<T> r;
 #pragma unroll
@@ -724,18 +734,18 @@ template <typename T> struct ConstIn {
 template <typename T, bool DAMAGE>
 __device__ __forceinline__ void load_const_in(const StepArgs<T>& A, long long p, ConstIn<T>& q) {
     q.d = A.d[p];
-    q.mu = A.mu[p];
-    q.lam = A.lam[p];
-    q.res = A.residual[p];
+    q.mu = fld<PMPM_U_MU>(A, A.mu, p);
+    q.lam = fld<PMPM_U_LAM>(A, A.lam, p);
+    q.res = fld<PMPM_U_RESIDUAL>(A, A.residual, p);
     q.iface = A.imask[p] != 0;
     q.Dh = A.D_hist[p];
     if constexpr (DAMAGE) {
-        q.A0 = load_m3(A.A0, p);
-        q.sc = A.sigma_c[p];
-        q.zeta = A.zeta[p];
-        q.l0 = A.l0[p];
+        q.A0 = ((A.umask >> PMPM_U_A0) & 1u) ? A.uA0 : load_m3(A.A0, p);
+        q.sc = fld<PMPM_U_SIGMA_C>(A, A.sigma_c, p);
+        q.zeta = fld<PMPM_U_ZETA>(A, A.zeta, p);
+        q.l0 = fld<PMPM_U_L0>(A, A.l0, p);
         q.lap = A.lap[p];
-        q.eta = A.eta[p];
+        q.eta = fld<PMPM_U_ETA>(A, A.eta, p);
         q.allow = A.allow[p] != 0;
     }
 }
@@ -967,7 +977,7 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     const M3<T> Cp = load_m3(A.C, p);
     T vv[3] = {A.v[3 * p], A.v[3 * p + 1], A.v[3 * p + 2]};
     const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
-    const T mass = A.mass[p], vol0 = A.vol0[p];
+    const T mass = fld<PMPM_U_MASS>(A, A.mass, p), vol0 = fld<PMPM_U_VOL0>(A, A.vol0, p);
     const int bc = A.bc_id ? A.bc_id[p] : 0;
     const long long i = A.inv[p];
     // a batched step gates damage per scene; the scene is in the cell key
@@ -1862,7 +1872,7 @@ __global__ void __launch_bounds__(WG) k_pcg_const(StepArgs<T> A) {
     ConstIn<T> q;
     load_const_in<T, true>(A, p, q);
     const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
-    const T l0 = A.l0[p];
+    const T l0 = q.l0;
     const long long i = A.inv[p];
     T d = q.d, J, Dh;
     constitutive<T, true, EXACT_SIGMA, true>(A, p, q, F, d, J, true, &Dh);
@@ -2565,6 +2575,10 @@ template <typename T> StepArgs<T> make_args(const pmpm_step_params* p, const pmp
     A.vol0 = (const T*)in->vol0; A.mass = (const T*)in->mass; A.mu = (const T*)in->mu; A.lam = (const T*)in->lam;
     A.sigma_c = (const T*)in->sigma_c; A.eta = (const T*)in->eta; A.zeta = (const T*)in->zeta; A.l0 = (const T*)in->l0;
     A.residual = (const T*)in->residual; A.A0 = (const T*)in->A0; A.n_if = (const T*)in->n_interface;
+    A.umask = in->uniform_mask & ((1u << 10) - 1u);
+    for (int k = 0; k < 9; ++k) A.uval[k] = T(in->uniform_value[k]);
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) A.uA0.m[a][b] = T(in->uniform_A0[3 * a + b]);
     A.allow = in->allow; A.imask = in->interface_mask; A.bc_id = in->bc_id;
     A.ox = (T*)out->x; A.ov = (T*)out->v; A.oF = (T*)out->F; A.oC = (T*)out->C; A.olap = (T*)out->lap;
     A.od = (T*)out->d; A.oD = (T*)out->D_hist; A.ophi = (T*)out->phi; A.ophs = (T*)out->phi_shear;
@@ -2638,15 +2652,20 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         if (rc) return rc;
     }
     if (in->n > 0) {
-        const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, in->vol0, in->mass, in->mu, in->lam,
-                              in->residual, in->interface_mask, out->x, out->v, out->F, out->C, out->lap};
+        // a material field given as a uniform value needs no array
+        auto fieldp = [&](const void* a, int k) { return ((in->uniform_mask >> k) & 1u) ? (const void*)in : a; };
+        const void* need[] = {in->x, in->v, in->F, in->C, in->d, in->lap, fieldp(in->vol0, PMPM_U_VOL0),
+                              fieldp(in->mass, PMPM_U_MASS), fieldp(in->mu, PMPM_U_MU), fieldp(in->lam, PMPM_U_LAM),
+                              fieldp(in->residual, PMPM_U_RESIDUAL), in->interface_mask, out->x, out->v, out->F, out->C,
+                              out->lap};
         for (const void* q : need)
             if (!q) return fail(PMPM_EARG, "%smissing particle array", "");
         if (in->n_interface == nullptr) return fail(PMPM_EARG, "%smissing n_interface", "");
         if (!out->tau_last != !out->J_last) return fail(PMPM_EARG, "%stau_last and J_last are skipped together", "");
         if (p->do_damage) {
-            const void* dm[] = {in->D_hist, in->sigma_c, in->eta, in->zeta, in->l0, in->A0, in->allow,
-                                out->d, out->D_hist};
+            const void* dm[] = {in->D_hist, fieldp(in->sigma_c, PMPM_U_SIGMA_C), fieldp(in->eta, PMPM_U_ETA),
+                                fieldp(in->zeta, PMPM_U_ZETA), fieldp(in->l0, PMPM_U_L0), fieldp(in->A0, PMPM_U_A0),
+                                in->allow, out->d, out->D_hist};
             for (const void* q : dm)
                 if (!q) return fail(PMPM_EARG, "%smissing damage array", "");
             if (!out->phi != !out->phi_shear) return fail(PMPM_EARG, "%sphi and phi_shear are skipped together", "");
@@ -3063,6 +3082,7 @@ int pmpm_substep_adjoint(const pmpm_step_params* p, const pmpm_particles_in* in,
                     (long long)pmpm_adjoint_workspace_bytes(in->n, p->n_grid, p->dtype));
     if (p->n_scenes > 1 || p->damage_solver != PMPM_DAMAGE_EXPLICIT || p->export_active)
         return fail(PMPM_EARG, "%sthe adjoint supports one scene, device colliders and the explicit damage update", "");
+    if (in->uniform_mask) return fail(PMPM_EARG, "%sthe adjoint takes per-particle material arrays (uniform_mask 0)", "");
     if (in->n > 0) {
         const void* need[] = {gin->x, gin->v, gin->F, gin->C, gin->d, gin->lap, gin->D_hist, gout->x, gout->v, gout->F,
                               gout->C, gout->d, gout->lap, gout->D_hist, gout->mu, gout->lam, gout->A0, gout->sigma_c, fwd_out->d,

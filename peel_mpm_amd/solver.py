@@ -36,7 +36,7 @@ import math
 
 import torch
 
-from . import _lib
+from . import _lib, uniform
 from .boundaries import DeviceBoundaryTable, HalfspaceCollider, SphereCollider, VelocityBC
 
 _STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
@@ -99,6 +99,7 @@ class AnisoMPM:
         # counted the next step's binning; the next step may skip its count
         # pass while x is still that tensor, unmodified
         self._carry = None
+        self._uf = uniform.UniformFields()
         self._pin = _lib.ParticlesIn()
         self._out_n = -1
         _lib.lib()
@@ -231,6 +232,17 @@ class AnisoMPM:
             self._bc_id, self._bc_key = ids, key
         return self._bc_id, [b.value(time) for b in bcs]
 
+    def _uniform_fields(self):
+        """(mask, values) of the material fields that hold one value on every
+        particle: the step reads that value instead of the array."""
+        return self._uf.state(self)
+
+    def _uniform_key(self):
+        return uniform.key(self._uniform_fields())
+
+    def _set_uniform(self, pin):
+        uniform.fill(pin, self._uniform_fields())
+
     def _config_key(self):
         fc = None if self.f_clamp is None else (float(self.f_clamp[0]), float(self.f_clamp[1]))
         return (self._code, self.n_grid, self.grid_lim, self.dx, self.inv_dx, self.dt, tuple(self._gravity_host),
@@ -327,6 +339,7 @@ class AnisoMPM:
         pin.allow = allow.data_ptr()
         pin.interface_mask = imask.data_ptr()
         pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
+        self._set_uniform(pin)
 
         if self._out_n != self.n:
             self._layout_outputs()
@@ -465,9 +478,12 @@ class AnisoMPM:
             G["pattern"][pk] = torch.tensor([(self._step + s) % self.damage_every == 0 for s in range(n_sub)],
                                             dtype=torch.uint8).to(dev)
         torch.mul(G["pattern"][pk], allow.any().to(torch.uint8), out=G["gate"][:n_sub])
+        # the uniform material values are kernel arguments of the captured
+        # launches: a change of them (an in-place edit) captures again
         key = (n_sub, cur, self._diag_substeps(n_sub), self._config_key(), len(self.colliders), len(self.particle_bc),
                bc_id.data_ptr() if bc_id is not None else 0, allow.data_ptr(), imask.data_ptr(), ws.data_ptr(),
-               tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS), tuple(_launch_shape()))
+               tuple(getattr(self, k).data_ptr() for k in _FLOAT_PARAMS), tuple(_launch_shape()),
+               self._uniform_key())
         g = G["graphs"].get(key)
         if g is None:
             G["graphs"] = {k2: v for k2, v in G["graphs"].items() if k2[3:] == key[3:]}
@@ -521,6 +537,7 @@ class AnisoMPM:
                 setattr(pin, k, getattr(self, k).data_ptr())
             pin.allow, pin.interface_mask = allow.data_ptr(), imask.data_ptr()
             pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
+            self._set_uniform(pin)
             pout = _lib.ParticlesOut()
             for k in _STATE:
                 setattr(pout, k, dst[k].data_ptr())

@@ -89,8 +89,8 @@ def test_diagnostic_outputs_are_skipped_in_pairs():
     p, pin, pout = _lib.StepParams(), _lib.ParticlesIn(), _lib.ParticlesOut()
     p.dtype, p.n_grid = _lib.PMPM_F32, 32
     pin.n = 10
-    for name, _ in _lib.ParticlesIn._fields_:
-        if name != "n" and name != "bc_id":
+    for name in _lib._IN_PTRS:
+        if name != "bc_id":
             setattr(pin, name, fake)
     for name, _ in _lib.ParticlesOut._fields_:
         setattr(pout, name, fake + 4096)
