@@ -25,41 +25,44 @@ class UniformFields:
     def __init__(self):
         self._keys: dict = {}
         self._vals: dict = {}
+        self._all = None
+        self._state = (0, {})
 
     def state(self, owner):
-        """(mask, {field: values}) with bit k of mask for field k."""
-        changed = []
-        for name in _lib.UNIFORM_FIELDS:
-            t = getattr(owner, name)
-            key = (id(t), t.data_ptr(), t._version, tuple(t.shape))
-            if self._keys.get(name) != key:
-                changed.append((name, t, key))
-        if changed:
-            flags, firsts = [], []
-            for name, t, _ in changed:
-                flat = t.reshape(t.shape[0], 1 if t.dim() == 1 else 9)
-                if flat.shape[0] == 0:
-                    flags.append(torch.zeros((), dtype=torch.bool, device=t.device))
-                    firsts.append(torch.zeros(flat.shape[1], dtype=torch.float64, device=t.device))
-                    continue
-                bits = flat.view(torch.int32 if t.element_size() == 4 else torch.int64)
-                flags.append((bits == bits[:1]).all())
-                firsts.append(flat[0].double())
-            ok = torch.stack(flags).tolist()
-            vals = torch.cat(firsts).tolist()
-            o = 0
-            for (name, t, key), u in zip(changed, ok):
-                w = 1 if t.dim() == 1 else 9
-                self._vals[name] = tuple(vals[o:o + w]) if u else None
-                self._keys[name] = key
-                o += w
-        mask, vals = 0, {}
+        """(mask, {field: values}) with bit k of mask for field k; the same
+        object while no field changed."""
+        ts = [getattr(owner, name) for name in _lib.UNIFORM_FIELDS]
+        allkeys = tuple((id(t), t.data_ptr(), t._version, t.shape[0]) for t in ts)
+        if allkeys == self._all:
+            return self._state
+        changed = [(name, t, key) for name, t, key in zip(_lib.UNIFORM_FIELDS, ts, allkeys)
+                   if self._keys.get(name) != key]
+        flags, firsts = [], []
+        for name, t, _ in changed:
+            flat = t.reshape(t.shape[0], 1 if t.dim() == 1 else 9)
+            if flat.shape[0] == 0:
+                flags.append(torch.zeros((), dtype=torch.bool, device=t.device))
+                firsts.append(torch.zeros(flat.shape[1], dtype=torch.float64, device=t.device))
+                continue
+            bits = flat.view(torch.int32 if t.element_size() == 4 else torch.int64)
+            flags.append((bits == bits[:1]).all())
+            firsts.append(flat[0].double())
+        ok = torch.stack(flags).tolist()
+        vals = torch.cat(firsts).tolist()
+        o = 0
+        for (name, t, key), u in zip(changed, ok):
+            w = 1 if t.dim() == 1 else 9
+            self._vals[name] = tuple(vals[o:o + w]) if u else None
+            self._keys[name] = key
+            o += w
+        mask, out = 0, {}
         for k, name in enumerate(_lib.UNIFORM_FIELDS):
             v = self._vals.get(name)
             if v is not None:
                 mask |= 1 << k
-                vals[name] = v
-        return mask, vals
+                out[name] = v
+        self._all, self._state = allkeys, (mask, out)
+        return self._state
 
 
 def key(state):
@@ -70,7 +73,11 @@ def key(state):
 
 
 def fill(pin, state):
-    """Write a (mask, values) state into a ParticlesIn."""
+    """Write a (mask, values) state into a ParticlesIn (skipped when this
+    pin already holds this very state object)."""
+    if getattr(pin, "_uniform_state", None) is state:
+        return
+    pin._uniform_state = state
     mask, vals = state
     pin.uniform_mask = mask
     for k, name in enumerate(_lib.UNIFORM_FIELDS):
