@@ -41,6 +41,7 @@ from .boundaries import DeviceBoundaryTable, HalfspaceCollider, SphereCollider, 
 
 _STATE = ("x", "v", "F", "C", "d", "lap", "D_hist")
 _FLOAT_PARAMS = ("vol0", "mass", "mu", "lam", "sigma_c", "eta", "zeta", "l0", "residual", "A0", "n_interface")
+_FLAGS = ("allow", "interface_mask")
 
 
 def _launch_shape():
@@ -101,6 +102,7 @@ class AnisoMPM:
         self._carry = None
         self._uf = uniform.UniformFields()
         self._pin = _lib.ParticlesIn()
+        self._checked = {}
         self._out_n = -1
         _lib.lib()
 
@@ -275,11 +277,11 @@ class AnisoMPM:
         of one allocation."""
         parts, total = self._out_parts_dmg if do_damage else self._out_parts
         flat = torch.empty(total, device=self.device, dtype=self.dtype)
-        return {k: flat[o:o + sz].view(shape) for k, shape, o, sz in parts}
+        return {k: flat.as_strided(shape, stride, o) for k, shape, stride, o in parts}
 
     def _layout_outputs(self):
-        """(name, shape, start, size) of each output in the one allocation,
-        each start 256-byte aligned in fp32."""
+        """(name, shape, contiguous strides, start) of each output in the one
+        allocation, each start 256-byte aligned in fp32."""
         n = self.n
         base = [("x", (n, 3)), ("v", (n, 3)), ("F", (n, 3, 3)), ("C", (n, 3, 3)), ("lap", (n,)),
                 ("tau_last", (n, 3, 3)), ("J_last", (n,))]
@@ -291,7 +293,8 @@ class AnisoMPM:
                 sz = 1
                 for d in shape:
                     sz *= d
-                parts.append((k, shape, o, sz))
+                stride = tuple(math.prod(shape[i + 1:]) for i in range(len(shape)))
+                parts.append((k, shape, stride, o))
                 o += (sz + 63) // 64 * 64
             return parts, max(o, 1)
         self._out_parts, self._out_parts_dmg = lay(base), lay(base + extra)
@@ -307,10 +310,18 @@ class AnisoMPM:
         if bc_id is None:
             for bc in self.particle_bc:
                 bc(self, time)
-        for k in _STATE + _FLOAT_PARAMS:
-            self._tensor(k)
-        allow = self._tensor("allow", torch.bool)
-        imask = self._tensor("interface_mask", torch.bool)
+        # device, dtype and layout checks, skipped for a tensor already
+        # checked (held in _checked, so identity cannot be a reused object)
+        # or written by the last step; their pointers go into pin once
+        pin = self._pin
+        chk = self._checked
+        for k in _STATE + _FLOAT_PARAMS + _FLAGS:
+            t = getattr(self, k)
+            if chk.get(k) is not t:
+                t = self._tensor(k, torch.bool if k in _FLAGS else None)
+                chk[k] = t
+                setattr(pin, k, t.data_ptr())
+        allow, imask = self.allow, self.interface_mask
         do_damage = self._any_allow() and (self._step % self.damage_every == 0)
 
         p = self._step_params()
@@ -332,12 +343,7 @@ class AnisoMPM:
         for k, v in enumerate(bc_vals):
             p.vel_bc[k][:] = v
 
-        pin = self._pin
         pin.n = self.n
-        for k in _STATE + _FLOAT_PARAMS:
-            setattr(pin, k, getattr(self, k).data_ptr())
-        pin.allow = allow.data_ptr()
-        pin.interface_mask = imask.data_ptr()
         pin.bc_id = bc_id.data_ptr() if bc_id is not None else None
         self._set_uniform(pin)
 
@@ -367,6 +373,10 @@ class AnisoMPM:
             _lib.check(L.pmpm_substep_end(*args), "pmpm_substep_end")
         for k, t in new.items():
             setattr(self, k, t)
+            if k in self._checked:
+                # the step's own output: contiguous, on the device, in dtype
+                self._checked[k] = t
+                setattr(self._pin, k, t.data_ptr())
         self._carry = (self.x.data_ptr(), self.x._version, self.n)
         self._step += 1
 
