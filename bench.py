@@ -35,6 +35,11 @@ Algorithmic bytes follow one fixed convention (fp32):
                application (the operator goes through the particles)
   diagnostics  phi, phi_shear, tau_last, J_last (48 B per particle written
                each step) are outside the convention and reported apart
+The convention counts every material field as a per-particle read.  A field
+that holds one value on every particle goes to the kernels as that value
+(config1: vol0, mass, sigma_c, eta, zeta, l0, residual), so 28 of its 73
+material bytes per particle are not moved; the measured ``traffic`` shows
+what is.
 ``roofline`` is for the dominant kernel group, timed live with HIP events
 recorded by libpmpm on the launch stream (pmpm_timing): its algorithmic
 bytes per launch / its mean launch time, against MI355X's 8 TB/s HBM peak.
@@ -354,7 +359,7 @@ def stiff_damage(sim):
 def time_small_frame(steps):
     """The host floor of a small scene: config4's scene (32,768 particles,
     fp32) stepped by run_frame eagerly (one substep call per step: argument
-    set-up plus ten kernel launches from the host) and replayed from a
+    set-up plus eight kernel launches from the host) and replayed from a
     captured graph (use_graph=True: one replay per frame), at the launch
     shape the autotuner picks for it.  Also the host time one eager
     substep call takes to enqueue its work."""
