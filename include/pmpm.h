@@ -269,6 +269,12 @@ typedef struct {
                                adjacent along z in one wavefront, their cells packed onto the
                                lanes by particle count, into one 7x7x11-node arena, which the
                                grid pass folds); default 4.  The PCG operator keeps 4. */
+    int32_t p2g_split;      /* P2G work split with cell_block 4: 1 (each cell on one lane,
+                               p2g_waves wavefronts per bucket sharing its particles) or 2
+                               (two wavefronts per bucket sharing its channels, quadratic and
+                               cubic, each with two arenas: a cell of two or more passes split
+                               over them and the halves packed onto the lanes by load;
+                               p2g_waves is then not used); default 1 */
 } pmpm_launch_config;
 /* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
 int pmpm_set_launch_config(const pmpm_launch_config *cfg);

@@ -121,7 +121,7 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
-pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4};
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1};
 
 // Workspace layout (byte offsets).  The step keeps count, bflag, mark and
 // misc at zero between steps: whoever reads them clears them.  count/start
@@ -1005,6 +1005,10 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     A.dsrt[i] = d;
 }
 
+// Channel sets of a P2G scatter: all six arena channels, the quadratic four
+// (m, mv) or the cubic two (sum w d, sum w).
+constexpr int CH_ALL = 0, CH_QUAD = 1, CH_CUBIC = 2;
+
 // One particle's P2G data in its cell's 4x4x4 frame: per axis and frame
 // node the quadratic weight W (zero-padded to the frame), D = W (x_i - x_p)
 // for that axis, and the cubic weight wc; the affine matrix, mass * v, mass
@@ -1014,21 +1018,24 @@ template <typename T> struct P2GPart {
     int bcb[3];
 };
 
-template <typename T>
+template <typename T, int CH = CH_ALL>
 __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int ng, P2GPart<T>& q, int sh[3]) {
     const T* r = A.rec + (size_t)P2G_REC * i;
     const T Xp[3] = {scaled(r[0], A.inv_dx), scaled(r[1], A.inv_dx), scaled(r[2], A.inv_dx)};
+    if constexpr (CH != CH_CUBIC) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int b = 0; b < 3; ++b) q.af[a][b] = r[3 + 3 * a + b];
-    q.mv[0] = r[12]; q.mv[1] = r[13]; q.mv[2] = r[14];
-    q.mass = r[15];
-    q.d = A.dsrt[i];
+            for (int b = 0; b < 3; ++b) q.af[a][b] = r[3 + 3 * a + b];
+        q.mv[0] = r[12]; q.mv[1] = r[13]; q.mv[2] = r[14];
+        q.mass = r[15];
+    }
+    if constexpr (CH != CH_QUAD) q.d = A.dsrt[i];
     int bq[3];
     T fx[3], w[3][3], w2[4][3];
     quad_weights(Xp, ng, bq, fx, w);
     cubic_weights(Xp, ng, q.bcb, q.wc, w2);
+    if constexpr (CH == CH_CUBIC) return;
     // the quadratic stencil [b, b+2] (b = c-1 or c) sits in the frame at shift sh
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -1043,13 +1050,16 @@ __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int 
 }
 
 // Zero contribution (a pair's second slot when it has no partner).
-template <typename T> __device__ __forceinline__ void p2g_mute(P2GPart<T>& q) {
-    q.mass = T(0);
-    q.d = T(0);
+template <typename T, int CH = CH_ALL> __device__ __forceinline__ void p2g_mute(P2GPart<T>& q) {
+    if constexpr (CH != CH_CUBIC) q.mass = T(0);
+    if constexpr (CH != CH_QUAD) q.d = T(0);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
 #pragma unroll
-        for (int a = 0; a < 3; ++a) { q.W[k][a] = T(0); q.D[k][a] = T(0); q.wc[k][a] = T(0); }
+        for (int a = 0; a < 3; ++a) {
+            if constexpr (CH != CH_CUBIC) { q.W[k][a] = T(0); q.D[k][a] = T(0); }
+            if constexpr (CH != CH_QUAD) q.wc[k][a] = T(0);
+        }
 }
 
 // Scatter NP particles that share one frame (lp: arena position of each
@@ -1068,13 +1078,14 @@ template <typename T> __device__ __forceinline__ void p2g_mute(P2GPart<T>& q) {
 // write (so after it, in LDS order), and the next node's contributions are
 // computed while that read is in flight; a scheduling barrier keeps that
 // arithmetic behind the read.
-template <typename T, int NP, int AZ = AR>
+template <typename T, int NP, int AZ = AR, int CH = CH_ALL>
 __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp[3][4], const P2GPart<T> (&q)[NP]) {
+    constexpr bool QD = CH != CH_CUBIC, CU = CH != CH_QUAD;
     T c0, c1, c2, c3, f0, f1;
     {
         const int l = (lp[0][0] * AR + lp[1][0]) * AZ + lp[2][0];
-        c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3];
-        f0 = sc[l][0]; f1 = sc[l][1];
+        if constexpr (QD) { c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3]; }
+        if constexpr (CU) { f0 = sc[l][0]; f1 = sc[l][1]; }
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1084,41 +1095,50 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
             T P[NP][3], Q[NP][3], M[NP], CW[NP], CD[NP];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
-                const T WW = q[j].W[ox][0] * q[j].W[oy][1];
-                const T DW = q[j].D[ox][0] * q[j].W[oy][1];
-                const T WD = q[j].W[ox][0] * q[j].D[oy][1];
+                if constexpr (QD) {
+                    const T WW = q[j].W[ox][0] * q[j].W[oy][1];
+                    const T DW = q[j].D[ox][0] * q[j].W[oy][1];
+                    const T WD = q[j].W[ox][0] * q[j].D[oy][1];
 #pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    P[j][c] = q[j].af[c][1] * WD + (q[j].af[c][0] * DW + WW * q[j].mv[c]);
-                    Q[j][c] = q[j].af[c][2] * WW;
+                    for (int c = 0; c < 3; ++c) {
+                        P[j][c] = q[j].af[c][1] * WD + (q[j].af[c][0] * DW + WW * q[j].mv[c]);
+                        Q[j][c] = q[j].af[c][2] * WW;
+                    }
+                    M[j] = WW * q[j].mass;
                 }
-                M[j] = WW * q[j].mass;
-                CW[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
-                CD[j] = CW[j] * q[j].d;
+                if constexpr (CU) {
+                    CW[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
+                    CD[j] = CW[j] * q[j].d;
+                }
             }
             const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AZ;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
 #pragma unroll
                 for (int j = 0; j < NP; ++j) {
-                    const T Wk = q[j].W[oz][2], Dk = q[j].D[oz][2], Ck = q[j].wc[oz][2];
-                    c0 = M[j] * Wk + c0;
-                    c1 = P[j][0] * Wk + (Q[j][0] * Dk + c1);
-                    c2 = P[j][1] * Wk + (Q[j][1] * Dk + c2);
-                    c3 = P[j][2] * Wk + (Q[j][2] * Dk + c3);
-                    f0 = CD[j] * Ck + f0;
-                    f1 = CW[j] * Ck + f1;
+                    if constexpr (QD) {
+                        const T Wk = q[j].W[oz][2], Dk = q[j].D[oz][2];
+                        c0 = M[j] * Wk + c0;
+                        c1 = P[j][0] * Wk + (Q[j][0] * Dk + c1);
+                        c2 = P[j][1] * Wk + (Q[j][1] * Dk + c2);
+                        c3 = P[j][2] * Wk + (Q[j][2] * Dk + c3);
+                    }
+                    if constexpr (CU) {
+                        const T Ck = q[j].wc[oz][2];
+                        f0 = CD[j] * Ck + f0;
+                        f1 = CW[j] * Ck + f1;
+                    }
                 }
                 const int l = lxy + lp[2][oz];
-                sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3;
-                sc[l][0] = f0; sc[l][1] = f1;
+                if constexpr (QD) { sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3; }
+                if constexpr (CU) { sc[l][0] = f0; sc[l][1] = f1; }
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
                 if (n < 64) {
                     const int nx = n >> 4, ny = (n >> 2) & 3, nz = n & 3;
                     const int ln = (lp[0][nx] * AR + lp[1][ny]) * AZ + lp[2][nz];
-                    c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3];
-                    f0 = sc[ln][0]; f1 = sc[ln][1];
+                    if constexpr (QD) { c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3]; }
+                    if constexpr (CU) { f0 = sc[ln][0]; f1 = sc[ln][1]; }
                 }
                 __builtin_amdgcn_sched_barrier(0);
             }
@@ -1128,33 +1148,36 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
 
 // One pass of a lane over its cell: the sorted particle record i (and i + 1
 // when NP is 2 and left > 1, left being the cell's records from i on)
-// scattered into the arenas sq, sc, whose node 0 is grid node org.
-template <typename T, int NP, int AZ = AR>
-__device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2]) {
+// scattered into the arenas sq, sc (the channels of CH), whose node 0 is
+// grid node org; ax offsets the arena position along x (arenas stacked
+// along x in one array).
+template <typename T, int NP, int AZ = AR, int CH = CH_ALL>
+__device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2],
+                                         int ax = 0) {
     const int ng = A.ng;
     if constexpr (NP == 1) {
         P2GPart<T> q1[1];
         int sh1[3], lp1[3][4];
-        p2g_load(A, i, ng, q1[0], sh1);
+        p2g_load<T, CH>(A, i, ng, q1[0], sh1);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
-        p2g_scatter<T, 1, AZ>(sq, sc, lp1, q1);
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
+        p2g_scatter<T, 1, AZ, CH>(sq, sc, lp1, q1);
         return;
     }
     P2GPart<T> q[NP];
     int sh[3];
 #pragma unroll
-    for (int j = 0; j < NP; ++j) p2g_load(A, i + (j < left ? j : 0), ng, q[j], sh);
+    for (int j = 0; j < NP; ++j) p2g_load<T, CH>(A, i + (j < left ? j : 0), ng, q[j], sh);
     // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as arena
     // positions (not grid-clamped; positions beyond the grid fold onto the
-    // edge node in the grid pass)
+    // edge node in the grid pass); ax offsets x (arenas stacked along x)
     int lp[3][4];
 #pragma unroll
     for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
+        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
     // particles of one cell share the frame unless they lie beyond the grid
     // (cells clamped together); such a particle goes alone after the others
     bool alone[NP];
@@ -1163,20 +1186,20 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
         const bool has = j < left;
         const bool same = has && q[j].bcb[0] == q[0].bcb[0] && q[j].bcb[1] == q[0].bcb[1] && q[j].bcb[2] == q[0].bcb[2];
         alone[j] = has && !same;
-        if (!same) p2g_mute(q[j]);
+        if (!same) p2g_mute<T, CH>(q[j]);
     }
-    p2g_scatter<T, NP, AZ>(sq, sc, lp, q);
+    p2g_scatter<T, NP, AZ, CH>(sq, sc, lp, q);
 #pragma unroll
     for (int j = 1; j < NP; ++j) {
         if (!alone[j]) continue;
         P2GPart<T> q1[1];
-        p2g_load(A, i + j, ng, q1[0], sh);
+        p2g_load<T, CH>(A, i + j, ng, q1[0], sh);
         int lp1[3][4];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1);
-        p2g_scatter<T, 1, AZ>(sq, sc, lp1, q1);
+            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
+        p2g_scatter<T, 1, AZ, CH>(sq, sc, lp1, q1);
     }
 }
 
@@ -1344,6 +1367,105 @@ k_p2g_cb8(StepArgs<T> A) {
         for (int l = lane; l < 6 * AR3B; l += 64) {
             const int node = l / 6, c = l - node * 6;
             dst[l] = c < 4 ? s_q[node][c] : s_c[node][c - 4];
+        }
+        __syncthreads();
+    }
+}
+
+// P2G split by channel, with cells split over two arenas: one work-group
+// of two wavefronts per 4x4x4-cell bucket, wavefront 0 scattering the
+// quadratic channels (m, mv) and wavefront 1 the cubic ones (sum w d,
+// sum w) of all the bucket's particles, each into its own pair of arenas.
+// With one arena per wavefront a bucket costs its fullest cell's passes
+// however empty the other lanes are, and a torn or compressed body leaves
+// cells unevenly filled (1 to 16 particles).  Here a cell of p >= 2 passes
+// is split: lane l runs the first a = ceil(p / 2) passes of its own cell l
+// into arena 0 (the lanes start in k_p2g's regular cell-per-lane pattern),
+// and the other p - a passes go, into arena 1, to a lane whose own load is
+// small: the lanes ranked by a ascending take the cells ranked by p - a
+// descending, rank for rank (ties by index, so the order is fixed).  A lane
+// runs its two items back to back.  Every item belongs to one lane, so at a
+// common frame offset the lanes of an instruction touch distinct entries:
+// distinct cells of one arena hit distinct nodes, and a cell's two items lie
+// in different arenas.  Splitting the channels between the wavefronts keeps
+// the work-group's LDS at one six-channel arena pair (two arenas per
+// wavefront at the size of one), and each wavefront's pass does its
+// channels' share of the arithmetic, and the smaller per-wavefront state
+// lets four wavefronts share a SIMD (128 VGPRs; k_p2g holds 168, three).
+// The arenas are summed in a fixed order and stored to the bucket's slot as
+// k_p2g stores it.
+template <typename T, int NP>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+k_p2g_chan(StepArgs<T> A) {
+    // arena pairs stacked along x: arena w's node (lx, ly, lz) is entry
+    // ((w * 7 + lx) * 7 + ly) * 7 + lz = w * 343 + node
+    __shared__ __align__(16) T s_q[2 * AR3][4];
+    __shared__ __align__(16) T s_c[2 * AR3][2];
+    __shared__ int s_item[2][128];          // per wavefront: second items by rank (first record, records)
+    const int nact = A.misc[0];
+    const int nb = A.nb;
+    for (BucketIter it(nact); it.i < it.end; it.i += it.step) {
+        // the lane's constants are recomputed per bucket (an opaque copy of
+        // the thread index), so the compiler keeps them out of the
+        // registers the passes need instead of spilling them
+        int tid = (int)threadIdx.x;
+        __asm__ __volatile__("" : "+v"(tid));
+        const int lane = tid & 63, wv = tid >> 6;
+        const unsigned long long below = (1ull << lane) - 1ull;
+        const int bk = A.alist[it.i];
+        const int loc = (int)(bk % A.NBs);
+        const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
+        const long long cell = ((long long)bk << 6) + lane;
+        const int c0 = A.start[cell], n = A.start[cell + 1] - c0;
+        const int pp = (n + NP - 1) / NP;
+        const int pa = pp >= 2 ? (pp + 1) >> 1 : pp, pb = pp - pa;
+        const int na = min(n, pa * NP);
+        int top = pa;
+        for (int o = 32; o > 0; o >>= 1) top = max(top, __shfl_xor(top, o));
+        // ranks: of pa ascending and of pb descending, ties by lane
+        int ra = 0, rb = 0, acc_a = 0, acc_b = 0;
+        for (int v = 0; v <= top; ++v) {
+            const unsigned long long ma = __ballot(pa == v), mb = __ballot(pb == top - v);
+            if (pa == v) ra = acc_a + __popcll(ma & below);
+            if (pb == top - v) rb = acc_b + __popcll(mb & below);
+            acc_a += __popcll(ma);
+            acc_b += __popcll(mb);
+        }
+        s_item[wv][rb] = c0 + na;
+        s_item[wv][64 + rb] = n - na;
+        // this wavefront's arena pair cleared
+        if (wv == 0) {
+            for (int l = lane; l < 2 * 4 * AR3; l += 64) (&s_q[0][0])[l] = T(0);
+        } else {
+            for (int l = lane; l < 2 * 2 * AR3; l += 64) (&s_c[0][0])[l] = T(0);
+        }
+        __syncthreads();
+        int ca = c0, nx = na, wa = 0;
+        const int cb = s_item[wv][ra], nbi = s_item[wv][64 + ra];
+        int rounds = pa + (nbi + NP - 1) / NP;
+        for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
+        const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
+        int r = 0, left = nbi;
+        for (int t = 0; t < rounds; ++t) {
+            if (r >= nx && left > 0) {
+                ca = cb;
+                nx = left;
+                wa = AR;
+                left = 0;
+                r = 0;
+            }
+            if (r < nx) {
+                if (wv == 0) p2g_pass<T, NP, AR, CH_QUAD>(A, ca + r, nx - r, org, s_q, nullptr, wa);
+                else p2g_pass<T, NP, AR, CH_CUBIC>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                r += NP;
+            }
+        }
+        __syncthreads();
+        // arena 0 + arena 1, node-major [343][6] as k_p2g stores it
+        T* dst = A.arena + (size_t)it.i * (6 * AR3);
+        for (int l = threadIdx.x; l < 6 * AR3; l += 128) {
+            const int node = l / 6, c = l - node * 6;
+            dst[l] = c < 4 ? s_q[node][c] + s_q[AR3 + node][c] : s_c[node][c - 4] + s_c[AR3 + node][c - 4];
         }
         __syncthreads();
     }
@@ -2507,6 +2629,11 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
         else hipLaunchKernelGGL((k_p2g_cb8<T, 2>), G, dim3(64), 0, st, A);
         return;
     }
+    if (g_launch.p2g_split == 2) {
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_chan<T, 1>), G, dim3(128), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g_chan<T, 2>), G, dim3(128), 0, st, A);
+        return;
+    }
     // four fp64 arenas would exceed 64 KiB of LDS per work-group: fp64 always takes two
     if constexpr (sizeof(T) == 4) {
         if (g_launch.p2g_waves == 4) {
@@ -2896,9 +3023,10 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
 }
 
 int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
-    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4};
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1};
     if (cfg) {
         if (cfg->cell_block) c.cell_block = cfg->cell_block;
+        if (cfg->p2g_split) c.p2g_split = cfg->p2g_split;
         if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
         if (cfg->block_wgs) c.block_wgs = cfg->block_wgs;
         if (cfg->p2g_pair) c.p2g_pair = cfg->p2g_pair;
@@ -2919,6 +3047,8 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
         return fail(PMPM_EARG, "%sg2p_threads must be 128, 256 or 512: %lld", "", c.g2p_threads);
     if (c.cell_block != 4 && c.cell_block != 8)
         return fail(PMPM_EARG, "%scell_block must be 4 or 8: %lld", "", c.cell_block);
+    if (c.p2g_split != 1 && c.p2g_split != 2)
+        return fail(PMPM_EARG, "%sp2g_split must be 1 or 2: %lld", "", c.p2g_split);
     g_launch = c;
     return PMPM_OK;
 }
