@@ -1062,8 +1062,11 @@ template <typename T, int CH = CH_ALL> __device__ __forceinline__ void p2g_mute(
         }
 }
 
-// Scatter NP particles that share one frame (lp: arena position of each
-// frame node per axis) into one wavefront's arena.  A particle adds to node
+// Scatter NP particles that share one frame into one wavefront's arena.
+// Node (i, j, k) of the frame sits at arena position l0 + (i AR + j) AZ + k
+// (l0 the frame's first node; the offsets are compile-time constants), or,
+// with CLAMP, at (lp[0][i] AR + lp[1][j]) AZ + lp[2][k] for a frame clamped
+// onto the arena's edge (a particle beyond the grid).  A particle adds to node
 // (i, j, k) of its frame
 //   m Wi Wj Wk,   Wi Wj Wk mv + A (Di Wj Wk, Wi Dj Wk, Wi Wj Dk)   (momentum)
 //   d wci wcj wck, wci wcj wck                                       (cubic)
@@ -1078,12 +1081,17 @@ template <typename T, int CH = CH_ALL> __device__ __forceinline__ void p2g_mute(
 // write (so after it, in LDS order), and the next node's contributions are
 // computed while that read is in flight; a scheduling barrier keeps that
 // arithmetic behind the read.
-template <typename T, int NP, int AZ = AR, int CH = CH_ALL>
-__device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp[3][4], const P2GPart<T> (&q)[NP]) {
+template <typename T, int NP, int AZ = AR, int CH = CH_ALL, bool CLAMP = false>
+__device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], int l0, const int (*lp)[4],
+                                            const P2GPart<T> (&q)[NP]) {
     constexpr bool QD = CH != CH_CUBIC, CU = CH != CH_QUAD;
+    auto at = [&](int ox, int oy, int oz) {
+        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AZ + lp[2][oz];
+        else return l0 + (ox * AR + oy) * AZ + oz;
+    };
     T c0, c1, c2, c3, f0, f1;
     {
-        const int l = (lp[0][0] * AR + lp[1][0]) * AZ + lp[2][0];
+        const int l = at(0, 0, 0);
         if constexpr (QD) { c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3]; }
         if constexpr (CU) { f0 = sc[l][0]; f1 = sc[l][1]; }
     }
@@ -1111,7 +1119,6 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
                     CD[j] = CW[j] * q[j].d;
                 }
             }
-            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AZ;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
 #pragma unroll
@@ -1129,14 +1136,14 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
                         f1 = CW[j] * Ck + f1;
                     }
                 }
-                const int l = lxy + lp[2][oz];
+                const int l = at(ox, oy, oz);
                 if constexpr (QD) { sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3; }
                 if constexpr (CU) { sc[l][0] = f0; sc[l][1] = f1; }
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
                 if (n < 64) {
                     const int nx = n >> 4, ny = (n >> 2) & 3, nz = n & 3;
-                    const int ln = (lp[0][nx] * AR + lp[1][ny]) * AZ + lp[2][nz];
+                    const int ln = at(nx, ny, nz);
                     if constexpr (QD) { c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3]; }
                     if constexpr (CU) { f0 = sc[ln][0]; f1 = sc[ln][1]; }
                 }
@@ -1144,6 +1151,35 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], const int lp
             }
         }
     }
+}
+
+// A particle's frame in the arena: its cubic base (cell - 1 per axis)
+// relative to the arena origin org sits at 0..3 (0..AZ-4 along z) for a
+// particle inside the grid, so the frame's 4 nodes per axis follow from its
+// first node (frame_origin; ax offsets x in stacked arenas).  A particle
+// beyond the grid is binned to the edge cell while its base lies further
+// out: its frame is clamped onto the arena's edge (frame_clamped true, node
+// positions per axis from frame_nodes), and the grid pass folds the edge
+// positions onto the edge nodes.
+template <int AZ>
+__device__ __forceinline__ bool frame_clamped(const int bcb[3], const int org[3]) {
+    const int x = bcb[0] - org[0], y = bcb[1] - org[1], z = bcb[2] - org[2];
+    return x < 0 || x > AR - 4 || y < 0 || y > AR - 4 || z < 0 || z > AZ - 4;
+}
+
+template <int AZ>
+__device__ __forceinline__ int frame_origin(const int bcb[3], const int org[3], int ax) {
+    const int x = clampi(bcb[0] - org[0], 0, AR - 4) + ax, y = clampi(bcb[1] - org[1], 0, AR - 4),
+              z = clampi(bcb[2] - org[2], 0, AZ - 4);
+    return (x * AR + y) * AZ + z;
+}
+
+template <int AZ>
+__device__ __forceinline__ void frame_nodes(const int bcb[3], const int org[3], int ax, int lp[3][4]) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
 }
 
 // One pass of a lane over its cell: the sorted particle record i (and i + 1
@@ -1155,51 +1191,35 @@ template <typename T, int NP, int AZ = AR, int CH = CH_ALL>
 __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2],
                                          int ax = 0) {
     const int ng = A.ng;
-    if constexpr (NP == 1) {
-        P2GPart<T> q1[1];
-        int sh1[3], lp1[3][4];
-        p2g_load<T, CH>(A, i, ng, q1[0], sh1);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
-        p2g_scatter<T, 1, AZ, CH>(sq, sc, lp1, q1);
-        return;
-    }
     P2GPart<T> q[NP];
     int sh[3];
 #pragma unroll
     for (int j = 0; j < NP; ++j) p2g_load<T, CH>(A, i + (j < left ? j : 0), ng, q[j], sh);
-    // the frame: the 4 nodes [c-1, c+2] per axis of the cell c, as arena
-    // positions (not grid-clamped; positions beyond the grid fold onto the
-    // edge node in the grid pass); ax offsets x (arenas stacked along x)
-    int lp[3][4];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
     // particles of one cell share the frame unless they lie beyond the grid
-    // (cells clamped together); such a particle goes alone after the others
+    // (cells clamped together); a particle beyond the grid (its frame
+    // clamped) goes alone after the others, with its frame's node positions
+    // clamped one by one
+    const bool clamped = frame_clamped<AZ>(q[0].bcb, org);
+    const int l0 = frame_origin<AZ>(q[0].bcb, org, ax);
     bool alone[NP];
+    alone[0] = clamped;
+    if (clamped) p2g_mute<T, CH>(q[0]);
 #pragma unroll
     for (int j = 1; j < NP; ++j) {
         const bool has = j < left;
         const bool same = has && q[j].bcb[0] == q[0].bcb[0] && q[j].bcb[1] == q[0].bcb[1] && q[j].bcb[2] == q[0].bcb[2];
-        alone[j] = has && !same;
-        if (!same) p2g_mute<T, CH>(q[j]);
+        alone[j] = has && (!same || clamped);
+        if (!same || clamped) p2g_mute<T, CH>(q[j]);
     }
-    p2g_scatter<T, NP, AZ, CH>(sq, sc, lp, q);
+    p2g_scatter<T, NP, AZ, CH>(sq, sc, l0, nullptr, q);
 #pragma unroll
-    for (int j = 1; j < NP; ++j) {
+    for (int j = 0; j < NP; ++j) {
         if (!alone[j]) continue;
         P2GPart<T> q1[1];
         p2g_load<T, CH>(A, i + j, ng, q1[0], sh);
         int lp1[3][4];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lp1[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, (a == 2 ? AZ : AR) - 1) + (a == 0 ? ax : 0);
-        p2g_scatter<T, 1, AZ, CH>(sq, sc, lp1, q1);
+        frame_nodes<AZ>(q1[0].bcb, org, ax, lp1);
+        p2g_scatter<T, 1, AZ, CH, true>(sq, sc, 0, lp1, q1);
     }
 }
 
@@ -2041,11 +2061,15 @@ template <typename T> __device__ __forceinline__ void pcg_mute(PcgPart<T>& q) {
 // are summed in registers, then one read-modify-write per node, in program
 // order, software-pipelined as the P2G scatter (the next node's entry is
 // read right after this node's write).
-template <typename T, int NP>
-__device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], const int lp[3][4], const PcgPart<T> (&q)[NP]) {
+template <typename T, int NP, bool CLAMP = false>
+__device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], int l0, const int (*lp)[4], const PcgPart<T> (&q)[NP]) {
+    auto at = [&](int ox, int oy, int oz) {
+        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
+        else return l0 + (ox * AR + oy) * AR + oz;
+    };
     T c0, c1, c2, c3;
     {
-        const int l = (lp[0][0] * AR + lp[1][0]) * AR + lp[2][0];
+        const int l = at(0, 0, 0);
         c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3];
     }
     __builtin_amdgcn_sched_barrier(0);
@@ -2060,7 +2084,6 @@ __device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], const int lp[3][4]
                 gxy[j] = q[j].ds[ox][0] * q[j].wc[oy][1];
                 gyx[j] = q[j].wc[ox][0] * q[j].ds[oy][1];
             }
-            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
                 T a0, a1, a2, a3;
@@ -2074,13 +2097,13 @@ __device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], const int lp[3][4]
                     if (j == 0) { a0 = t0; a1 = t1; a2 = t2; a3 = nw; }
                     else { a0 += t0; a1 += t1; a2 += t2; a3 += nw; }
                 }
-                const int l = lxy + lp[2][oz];
+                const int l = at(ox, oy, oz);
                 c0 += a0; c1 += a1; c2 += a2; c3 += a3;
                 sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3;
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
                 if (n < 64) {
-                    const int ln = (lp[0][n >> 4] * AR + lp[1][(n >> 2) & 3]) * AR + lp[2][n & 3];
+                    const int ln = at(n >> 4, (n >> 2) & 3, n & 3);
                     c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3];
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -2094,8 +2117,12 @@ __device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], const int lp[3][4]
 // arena, software-pipelined as above.  Both halves are separable: per
 // (ox, oy) the gather keeps sum_z N_z p and sum_z N'_z p, and the scatter
 // forms N_z (N'_x N_y g_x + N_x N'_y g_y) + N'_z (N_x N_y g_z).
-template <typename T, int NP>
-__device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, const int lp[3][4], const PcgPart<T> (&q)[NP]) {
+template <typename T, int NP, bool CLAMP = false>
+__device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, int l0, const int (*lp)[4], const PcgPart<T> (&q)[NP]) {
+    auto at = [&](int ox, int oy, int oz) {
+        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
+        else return l0 + (ox * AR + oy) * AR + oz;
+    };
     T g[NP][3];
 #pragma unroll
     for (int j = 0; j < NP; ++j) g[j][0] = g[j][1] = g[j][2] = T(0);
@@ -2103,10 +2130,9 @@ __device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, const int 
     for (int ox = 0; ox < 4; ++ox)
 #pragma unroll
         for (int oy = 0; oy < 4; ++oy) {
-            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
             T pz[4];
 #pragma unroll
-            for (int oz = 0; oz < 4; ++oz) pz[oz] = sp[lxy + lp[2][oz]];
+            for (int oz = 0; oz < 4; ++oz) pz[oz] = sp[at(ox, oy, oz)];
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
                 T s = q[j].wc[0][2] * pz[0], t = q[j].ds[0][2] * pz[0];
@@ -2121,7 +2147,7 @@ __device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, const int 
         }
 #pragma unroll
     for (int j = 0; j < NP; ++j) { g[j][0] *= q[j].kg; g[j][1] *= q[j].kg; g[j][2] *= q[j].kg; }
-    T c = sa[(lp[0][0] * AR + lp[1][0]) * AR + lp[2][0]];
+    T c = sa[at(0, 0, 0)];
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ox = 0; ox < 4; ++ox)
@@ -2135,48 +2161,47 @@ __device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, const int 
                 u[j] = q[j].ds[ox][0] * (q[j].wc[oy][1] * g[j][0]) + q[j].wc[ox][0] * (q[j].ds[oy][1] * g[j][1]);
                 v[j] = q[j].wc[ox][0] * (q[j].wc[oy][1] * g[j][2]);
             }
-            const int lxy = (lp[0][ox] * AR + lp[1][oy]) * AR;
 #pragma unroll
             for (int oz = 0; oz < 4; ++oz) {
                 T a = q[0].wc[oz][2] * u[0] + q[0].ds[oz][2] * v[0];
 #pragma unroll
                 for (int j = 1; j < NP; ++j) a += q[j].wc[oz][2] * u[j] + q[j].ds[oz][2] * v[j];
                 c += a;
-                sa[lxy + lp[2][oz]] = c;
+                sa[at(ox, oy, oz)] = c;
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
-                if (n < 64) c = sa[(lp[0][n >> 4] * AR + lp[1][(n >> 2) & 3]) * AR + lp[2][n & 3]];
+                if (n < 64) c = sa[at(n >> 4, (n >> 2) & 3, n & 3)];
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
 }
 
 // One wavefront's pass over a cell's particles r, r + 1 (pairs that share a
-// frame; a second particle beyond the grid, whose clamped frame differs, is
-// scattered alone).
+// frame).  A particle beyond the grid (its frame clamped onto the arena's
+// edge, which a second particle of the cell need not share) is scattered
+// alone afterwards, with its frame's node positions clamped one by one.
 template <typename T, bool APPLY>
 __device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, bool two, const int org[3], const T* sp, T* arena) {
     PcgPart<T> q[2];
     pcg_load<T, !APPLY>(A, i0, A.ng, q[0]);
     pcg_load<T, !APPLY>(A, i0 + (two ? 1 : 0), A.ng, q[1]);
-    int lp[3][4];
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q[0].bcb[a] + k - org[a], 0, AR - 1);
+    const bool clamped = frame_clamped<AR>(q[0].bcb, org);
     const bool same = two && q[1].bcb[0] == q[0].bcb[0] && q[1].bcb[1] == q[0].bcb[1] && q[1].bcb[2] == q[0].bcb[2];
-    if (!same) pcg_mute(q[1]);
-    if (APPLY) pcg_apply_scatter<T, 2>(sp, arena, lp, q);
-    else pcg_drive_scatter<T, 2>((T(*)[4])arena, lp, q);
-    if (two && !same) {
+    const bool alone[2] = {clamped, two && (!same || clamped)};
+    if (clamped) pcg_mute(q[0]);
+    if (!same || clamped) pcg_mute(q[1]);
+    const int l0 = frame_origin<AR>(q[0].bcb, org, 0);
+    if (APPLY) pcg_apply_scatter<T, 2>(sp, arena, l0, nullptr, q);
+    else pcg_drive_scatter<T, 2>((T(*)[4])arena, l0, nullptr, q);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        if (!alone[j]) continue;
         PcgPart<T> q1[1];
-        pcg_load<T, !APPLY>(A, i0 + 1, A.ng, q1[0]);
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) lp[a][k] = clampi(q1[0].bcb[a] + k - org[a], 0, AR - 1);
-        if (APPLY) pcg_apply_scatter<T, 1>(sp, arena, lp, q1);
-        else pcg_drive_scatter<T, 1>((T(*)[4])arena, lp, q1);
+        pcg_load<T, !APPLY>(A, i0 + j, A.ng, q1[0]);
+        int lp[3][4];
+        frame_nodes<AR>(q1[0].bcb, org, 0, lp);
+        if (APPLY) pcg_apply_scatter<T, 1, true>(sp, arena, 0, lp, q1);
+        else pcg_drive_scatter<T, 1, true>((T(*)[4])arena, 0, lp, q1);
     }
 }
 
