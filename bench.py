@@ -75,11 +75,13 @@ PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage so
 # at: the fastest over config1's whole 500-step run in a coordinate sweep of
 # the autotuner's knobs.  autotune.tune_solver on the scene's first steps
 # picks p2g_waves 1 (the intact slab's uniform buckets); over the run, two
-# wavefronts per bucket measured 0.272 against 0.286 ms per step, and the
-# two wavefronts split by channel (p2g_split 2, p2g_waves then unused)
-# 0.2607 against 0.2625 ms.
+# wavefronts per bucket measured 0.272 against 0.286 ms per step, the two
+# wavefronts split by channel (p2g_split 2, p2g_waves then unused) 0.2607
+# against 0.2625 ms, and the balanced channel split (p2g_split 3: mv_x, mv_y
+# on one wavefront, m, mv_z and the cubic pair on the other) 0.2528 against
+# 0.2559 ms.
 PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 2,
-                 "g2p_threads": 256, "cell_block": 4, "p2g_split": 2}
+                 "g2p_threads": 256, "cell_block": 4, "p2g_split": 3}
 
 FRAME = 10                          # substeps per run_frame call in the timed legs
 

@@ -270,11 +270,13 @@ typedef struct {
                                lanes by particle count, into one 7x7x11-node arena, which the
                                grid pass folds); default 4.  The PCG operator keeps 4. */
     int32_t p2g_split;      /* P2G work split with cell_block 4: 1 (each cell on one lane,
-                               p2g_waves wavefronts per bucket sharing its particles) or 2
+                               p2g_waves wavefronts per bucket sharing its particles), 2
                                (two wavefronts per bucket sharing its channels, quadratic and
                                cubic, each with two arenas: a cell of two or more passes split
                                over them and the halves packed onto the lanes by load;
-                               p2g_waves is then not used); default 1 */
+                               p2g_waves is then not used) or 3 (as 2, with the channels
+                               balanced between the wavefronts: mv_x, mv_y on one and m, mv_z
+                               and the cubic pair on the other); default 1 */
 } pmpm_launch_config;
 /* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
 int pmpm_set_launch_config(const pmpm_launch_config *cfg);

@@ -22,10 +22,11 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   lanes by particle count, into one 7x7x11-node arena; 21 % fewer arena
   nodes written and folded);
 * ``p2g_split``: how a P2G work-group's wavefronts share a 4x4x4-cell
-  bucket -- 1 (by particles: each takes every other pair of each cell) or
-  2 (by channels: one the quadratic, one the cubic, each with two arenas,
+  bucket -- 1 (by particles: each takes every other pair of each cell), 2
+  (by channels: one the quadratic, one the cubic, each with two arenas,
   the cells of two or more passes split over them and packed onto the
-  lanes by load).
+  lanes by load) or 3 (as 2 with the channels balanced: mv_x and mv_y on
+  one wavefront, m, mv_z and the cubic pair on the other).
 
 The bucket (4x4x4 cells, one cell per lane of a 64-wide wavefront) stays
 the unit of the binning and of the lane-per-cell scatter.
@@ -55,7 +56,7 @@ SPACE = {
     "const_threads": (64, 128, 256),
     "block_wgs": (512, 1024, 2048),
     "cell_block": (4, 8),
-    "p2g_split": (1, 2),
+    "p2g_split": (1, 2, 3),
 }
 DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2,
            "g2p_threads": 256, "cell_block": 4, "p2g_split": 1}

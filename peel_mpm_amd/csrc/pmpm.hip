@@ -1008,6 +1008,18 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
 // Channel sets of a P2G scatter: all six arena channels, the quadratic four
 // (m, mv) or the cubic two (sum w d, sum w).
 constexpr int CH_ALL = 0, CH_QUAD = 1, CH_CUBIC = 2;
+// The balanced pair of channel sets: (mv_x, mv_y) and (m, mv_z, sum w d,
+// sum w), about equal arithmetic per particle.
+constexpr int CH_XY = 3, CH_MZC = 4;
+
+// What a channel set scatters: the mass, momentum components [mlo, mhi),
+// the cubic pair.
+template <int CH> struct P2GChannels {
+    static constexpr bool mass = CH == CH_ALL || CH == CH_QUAD || CH == CH_MZC;
+    static constexpr int mlo = CH == CH_MZC ? 2 : 0;
+    static constexpr int mhi = CH == CH_CUBIC ? 0 : (CH == CH_XY ? 2 : 3);
+    static constexpr bool cubic = CH == CH_ALL || CH == CH_CUBIC || CH == CH_MZC;
+};
 
 // One particle's P2G data in its cell's 4x4x4 frame: per axis and frame
 // node the quadratic weight W (zero-padded to the frame), D = W (x_i - x_p)
@@ -1030,7 +1042,7 @@ __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int 
         q.mv[0] = r[12]; q.mv[1] = r[13]; q.mv[2] = r[14];
         q.mass = r[15];
     }
-    if constexpr (CH != CH_QUAD) q.d = A.dsrt[i];
+    if constexpr (P2GChannels<CH>::cubic) q.d = A.dsrt[i];
     int bq[3];
     T fx[3], w[3][3], w2[4][3];
     quad_weights(Xp, ng, bq, fx, w);
@@ -1062,10 +1074,11 @@ template <typename T, int CH = CH_ALL> __device__ __forceinline__ void p2g_mute(
         }
 }
 
-// Scatter NP particles that share one frame into one wavefront's arena.
-// Node (i, j, k) of the frame sits at arena position l0 + (i AR + j) AZ + k
+// Scatter NP particles that share one frame into one wavefront's arena,
+// whose node (x, y, z) is entry x XS + y AZ + z (XS = AR AZ dense, or
+// padded).  Node (i, j, k) of the frame sits at entry l0 + i XS + j AZ + k
 // (l0 the frame's first node; the offsets are compile-time constants), or,
-// with CLAMP, at (lp[0][i] AR + lp[1][j]) AZ + lp[2][k] for a frame clamped
+// with CLAMP, at lp[0][i] XS + lp[1][j] AZ + lp[2][k] for a frame clamped
 // onto the arena's edge (a particle beyond the grid).  A particle adds to node
 // (i, j, k) of its frame
 //   m Wi Wj Wk,   Wi Wj Wk mv + A (Di Wj Wk, Wi Dj Wk, Wi Wj Dk)   (momentum)
@@ -1081,20 +1094,36 @@ template <typename T, int CH = CH_ALL> __device__ __forceinline__ void p2g_mute(
 // write (so after it, in LDS order), and the next node's contributions are
 // computed while that read is in flight; a scheduling barrier keeps that
 // arithmetic behind the read.
-template <typename T, int NP, int AZ = AR, int CH = CH_ALL, bool CLAMP = false>
+template <typename T, int NP, int AZ = AR, int CH = CH_ALL, bool CLAMP = false, int XS = AR * AZ>
 __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], int l0, const int (*lp)[4],
                                             const P2GPart<T> (&q)[NP]) {
-    constexpr bool QD = CH != CH_CUBIC, CU = CH != CH_QUAD;
+    using K = P2GChannels<CH>;
+    constexpr bool QD = K::mass || K::mhi > K::mlo, CU = K::cubic;
     auto at = [&](int ox, int oy, int oz) {
-        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AZ + lp[2][oz];
-        else return l0 + (ox * AR + oy) * AZ + oz;
+        if constexpr (CLAMP) return lp[0][ox] * XS + lp[1][oy] * AZ + lp[2][oz];
+        else return l0 + ox * XS + oy * AZ + oz;
     };
+    // accumulators: c0 mass, c1..c3 momentum, f0 f1 cubic; the arena entry
+    // holds the set's channels (CH_XY: (mv_x, mv_y) in sc; CH_MZC: (m,
+    // mv_z, sum w d, sum w) in sq)
     T c0, c1, c2, c3, f0, f1;
-    {
-        const int l = at(0, 0, 0);
-        if constexpr (QD) { c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3]; }
-        if constexpr (CU) { f0 = sc[l][0]; f1 = sc[l][1]; }
-    }
+    auto load = [&](int l) {
+        if constexpr (CH == CH_XY) { c1 = sc[l][0]; c2 = sc[l][1]; }
+        else if constexpr (CH == CH_MZC) { c0 = sq[l][0]; c3 = sq[l][1]; f0 = sq[l][2]; f1 = sq[l][3]; }
+        else {
+            if constexpr (QD) { c0 = sq[l][0]; c1 = sq[l][1]; c2 = sq[l][2]; c3 = sq[l][3]; }
+            if constexpr (CU) { f0 = sc[l][0]; f1 = sc[l][1]; }
+        }
+    };
+    auto store = [&](int l) {
+        if constexpr (CH == CH_XY) { sc[l][0] = c1; sc[l][1] = c2; }
+        else if constexpr (CH == CH_MZC) { sq[l][0] = c0; sq[l][1] = c3; sq[l][2] = f0; sq[l][3] = f1; }
+        else {
+            if constexpr (QD) { sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3; }
+            if constexpr (CU) { sc[l][0] = f0; sc[l][1] = f1; }
+        }
+    };
+    load(at(0, 0, 0));
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int ox = 0; ox < 4; ++ox) {
@@ -1108,11 +1137,11 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], int l0, cons
                     const T DW = q[j].D[ox][0] * q[j].W[oy][1];
                     const T WD = q[j].W[ox][0] * q[j].D[oy][1];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
+                    for (int c = K::mlo; c < K::mhi; ++c) {
                         P[j][c] = q[j].af[c][1] * WD + (q[j].af[c][0] * DW + WW * q[j].mv[c]);
                         Q[j][c] = q[j].af[c][2] * WW;
                     }
-                    M[j] = WW * q[j].mass;
+                    if constexpr (K::mass) M[j] = WW * q[j].mass;
                 }
                 if constexpr (CU) {
                     CW[j] = q[j].wc[ox][0] * q[j].wc[oy][1];
@@ -1125,10 +1154,10 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], int l0, cons
                 for (int j = 0; j < NP; ++j) {
                     if constexpr (QD) {
                         const T Wk = q[j].W[oz][2], Dk = q[j].D[oz][2];
-                        c0 = M[j] * Wk + c0;
-                        c1 = P[j][0] * Wk + (Q[j][0] * Dk + c1);
-                        c2 = P[j][1] * Wk + (Q[j][1] * Dk + c2);
-                        c3 = P[j][2] * Wk + (Q[j][2] * Dk + c3);
+                        if constexpr (K::mass) c0 = M[j] * Wk + c0;
+                        if constexpr (K::mlo <= 0 && K::mhi > 0) c1 = P[j][0] * Wk + (Q[j][0] * Dk + c1);
+                        if constexpr (K::mlo <= 1 && K::mhi > 1) c2 = P[j][1] * Wk + (Q[j][1] * Dk + c2);
+                        if constexpr (K::mlo <= 2 && K::mhi > 2) c3 = P[j][2] * Wk + (Q[j][2] * Dk + c3);
                     }
                     if constexpr (CU) {
                         const T Ck = q[j].wc[oz][2];
@@ -1136,17 +1165,10 @@ __device__ __forceinline__ void p2g_scatter(T (*sq)[4], T (*sc)[2], int l0, cons
                         f1 = CW[j] * Ck + f1;
                     }
                 }
-                const int l = at(ox, oy, oz);
-                if constexpr (QD) { sq[l][0] = c0; sq[l][1] = c1; sq[l][2] = c2; sq[l][3] = c3; }
-                if constexpr (CU) { sc[l][0] = f0; sc[l][1] = f1; }
+                store(at(ox, oy, oz));
                 __asm__ __volatile__("" ::: "memory");
                 const int n = (ox * 4 + oy) * 4 + oz + 1;
-                if (n < 64) {
-                    const int nx = n >> 4, ny = (n >> 2) & 3, nz = n & 3;
-                    const int ln = at(nx, ny, nz);
-                    if constexpr (QD) { c0 = sq[ln][0]; c1 = sq[ln][1]; c2 = sq[ln][2]; c3 = sq[ln][3]; }
-                    if constexpr (CU) { f0 = sc[ln][0]; f1 = sc[ln][1]; }
-                }
+                if (n < 64) load(at(n >> 4, (n >> 2) & 3, n & 3));
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -1167,11 +1189,11 @@ __device__ __forceinline__ bool frame_clamped(const int bcb[3], const int org[3]
     return x < 0 || x > AR - 4 || y < 0 || y > AR - 4 || z < 0 || z > AZ - 4;
 }
 
-template <int AZ>
+template <int AZ, int XS = AR * AZ>
 __device__ __forceinline__ int frame_origin(const int bcb[3], const int org[3], int ax) {
     const int x = clampi(bcb[0] - org[0], 0, AR - 4) + ax, y = clampi(bcb[1] - org[1], 0, AR - 4),
               z = clampi(bcb[2] - org[2], 0, AZ - 4);
-    return (x * AR + y) * AZ + z;
+    return x * XS + y * AZ + z;
 }
 
 template <int AZ>
@@ -1187,7 +1209,7 @@ __device__ __forceinline__ void frame_nodes(const int bcb[3], const int org[3], 
 // scattered into the arenas sq, sc (the channels of CH), whose node 0 is
 // grid node org; ax offsets the arena position along x (arenas stacked
 // along x in one array).
-template <typename T, int NP, int AZ = AR, int CH = CH_ALL>
+template <typename T, int NP, int AZ = AR, int CH = CH_ALL, int XS = AR * AZ>
 __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, const int org[3], T (*sq)[4], T (*sc)[2],
                                          int ax = 0) {
     const int ng = A.ng;
@@ -1200,7 +1222,7 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
     // clamped) goes alone after the others, with its frame's node positions
     // clamped one by one
     const bool clamped = frame_clamped<AZ>(q[0].bcb, org);
-    const int l0 = frame_origin<AZ>(q[0].bcb, org, ax);
+    const int l0 = frame_origin<AZ, XS>(q[0].bcb, org, ax);
     bool alone[NP];
     alone[0] = clamped;
     if (clamped) p2g_mute<T, CH>(q[0]);
@@ -1211,7 +1233,7 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
         alone[j] = has && (!same || clamped);
         if (!same || clamped) p2g_mute<T, CH>(q[j]);
     }
-    p2g_scatter<T, NP, AZ, CH>(sq, sc, l0, nullptr, q);
+    p2g_scatter<T, NP, AZ, CH, false, XS>(sq, sc, l0, nullptr, q);
 #pragma unroll
     for (int j = 0; j < NP; ++j) {
         if (!alone[j]) continue;
@@ -1219,7 +1241,7 @@ __device__ __forceinline__ void p2g_pass(const StepArgs<T>& A, int i, int left, 
         p2g_load<T, CH>(A, i + j, ng, q1[0], sh);
         int lp1[3][4];
         frame_nodes<AZ>(q1[0].bcb, org, ax, lp1);
-        p2g_scatter<T, 1, AZ, CH, true>(sq, sc, 0, lp1, q1);
+        p2g_scatter<T, 1, AZ, CH, true, XS>(sq, sc, 0, lp1, q1);
     }
 }
 
@@ -1414,13 +1436,23 @@ k_p2g_cb8(StepArgs<T> A) {
 // lets four wavefronts share a SIMD (128 VGPRs; k_p2g holds 168, three).
 // The arenas are summed in a fixed order and stored to the bucket's slot as
 // k_p2g stores it.
-template <typename T, int NP>
+// The arenas' x stride is padded from 49 to CHAN_XS = 52 entries and lane l
+// owns cell (x, y, z) = ((l >> 2) & 3, l >> 4, l & 3): the 8 lanes of one
+// 128-byte LDS cycle of a 16-byte access (fixed y, x in {2k, 2k+1}, z 0..3)
+// then reach 8 distinct entries mod 8, and the 16 lanes of an 8-byte
+// access 16 distinct entries mod 16, so a frame offset's update of the
+// lanes' own cells runs without bank conflicts (the dense stride, in cell
+// order, conflicts two ways).
+constexpr int CHAN_XS = AR * AR + 3;
+constexpr int CHAN_E = 2 * AR * CHAN_XS;  // entries of an arena pair
+
+template <typename T, int NP, bool BAL>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 k_p2g_chan(StepArgs<T> A) {
     // arena pairs stacked along x: arena w's node (lx, ly, lz) is entry
-    // ((w * 7 + lx) * 7 + ly) * 7 + lz = w * 343 + node
-    __shared__ __align__(16) T s_q[2 * AR3][4];
-    __shared__ __align__(16) T s_c[2 * AR3][2];
+    // (w * 7 + lx) * CHAN_XS + ly * 7 + lz
+    __shared__ __align__(16) T s_q[CHAN_E][4];
+    __shared__ __align__(16) T s_c[CHAN_E][2];
     __shared__ int s_item[2][128];          // per wavefront: second items by rank (first record, records)
     const int nact = A.misc[0];
     const int nb = A.nb;
@@ -1435,7 +1467,7 @@ k_p2g_chan(StepArgs<T> A) {
         const int bk = A.alist[it.i];
         const int loc = (int)(bk % A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
-        const long long cell = ((long long)bk << 6) + lane;
+        const long long cell = ((long long)bk << 6) + ((((lane >> 2) & 3) << 4) | ((lane >> 4) << 2) | (lane & 3));
         const int c0 = A.start[cell], n = A.start[cell + 1] - c0;
         const int pp = (n + NP - 1) / NP;
         const int pa = pp >= 2 ? (pp + 1) >> 1 : pp, pb = pp - pa;
@@ -1455,9 +1487,9 @@ k_p2g_chan(StepArgs<T> A) {
         s_item[wv][64 + rb] = n - na;
         // this wavefront's arena pair cleared
         if (wv == 0) {
-            for (int l = lane; l < 2 * 4 * AR3; l += 64) (&s_q[0][0])[l] = T(0);
+            for (int l = lane; l < 4 * CHAN_E; l += 64) (&s_q[0][0])[l] = T(0);
         } else {
-            for (int l = lane; l < 2 * 2 * AR3; l += 64) (&s_c[0][0])[l] = T(0);
+            for (int l = lane; l < 2 * CHAN_E; l += 64) (&s_c[0][0])[l] = T(0);
         }
         __syncthreads();
         int ca = c0, nx = na, wa = 0;
@@ -1475,8 +1507,13 @@ k_p2g_chan(StepArgs<T> A) {
                 r = 0;
             }
             if (r < nx) {
-                if (wv == 0) p2g_pass<T, NP, AR, CH_QUAD>(A, ca + r, nx - r, org, s_q, nullptr, wa);
-                else p2g_pass<T, NP, AR, CH_CUBIC>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                if constexpr (BAL) {
+                    if (wv == 0) p2g_pass<T, NP, AR, CH_XY, CHAN_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                    else p2g_pass<T, NP, AR, CH_MZC, CHAN_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
+                } else {
+                    if (wv == 0) p2g_pass<T, NP, AR, CH_QUAD, CHAN_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
+                    else p2g_pass<T, NP, AR, CH_CUBIC, CHAN_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                }
                 r += NP;
             }
         }
@@ -1485,7 +1522,14 @@ k_p2g_chan(StepArgs<T> A) {
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < 6 * AR3; l += 128) {
             const int node = l / 6, c = l - node * 6;
-            dst[l] = c < 4 ? s_q[node][c] + s_q[AR3 + node][c] : s_c[node][c - 4] + s_c[AR3 + node][c - 4];
+            const int lx = node / (AR * AR), e0 = lx * CHAN_XS + (node - lx * AR * AR), e1 = e0 + AR * CHAN_XS;
+            if constexpr (BAL) {
+                // (m, mv_z, sum w d, sum w) in s_q, (mv_x, mv_y) in s_c
+                const int e = c == 0 ? 0 : (c >= 3 ? c - 2 : c - 1);
+                dst[l] = c == 1 || c == 2 ? s_c[e0][e] + s_c[e1][e] : s_q[e0][e] + s_q[e1][e];
+            } else {
+                dst[l] = c < 4 ? s_q[e0][c] + s_q[e1][c] : s_c[e0][c - 4] + s_c[e1][c - 4];
+            }
         }
         __syncthreads();
     }
@@ -2655,8 +2699,13 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
         return;
     }
     if (g_launch.p2g_split == 2) {
-        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_chan<T, 1>), G, dim3(128), 0, st, A);
-        else hipLaunchKernelGGL((k_p2g_chan<T, 2>), G, dim3(128), 0, st, A);
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_chan<T, 1, false>), G, dim3(128), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g_chan<T, 2, false>), G, dim3(128), 0, st, A);
+        return;
+    }
+    if (g_launch.p2g_split == 3) {
+        if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_chan<T, 1, true>), G, dim3(128), 0, st, A);
+        else hipLaunchKernelGGL((k_p2g_chan<T, 2, true>), G, dim3(128), 0, st, A);
         return;
     }
     // four fp64 arenas would exceed 64 KiB of LDS per work-group: fp64 always takes two
@@ -2679,7 +2728,7 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
 }
 
 // The binning.  pre: the carried count of the previous step's G2P (count2,
-// bflag2, and the keys and slots it wrote) is this x's, so the count pass is
+// btot2, tiles2, and the keys and slots it wrote) is this x's, so the count pass is
 // skipped; otherwise k_bin_count counts afresh and the scan also clears the
 // carried set.
 template <typename T>
@@ -3072,8 +3121,8 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
         return fail(PMPM_EARG, "%sg2p_threads must be 128, 256 or 512: %lld", "", c.g2p_threads);
     if (c.cell_block != 4 && c.cell_block != 8)
         return fail(PMPM_EARG, "%scell_block must be 4 or 8: %lld", "", c.cell_block);
-    if (c.p2g_split != 1 && c.p2g_split != 2)
-        return fail(PMPM_EARG, "%sp2g_split must be 1 or 2: %lld", "", c.p2g_split);
+    if (c.p2g_split < 1 || c.p2g_split > 3)
+        return fail(PMPM_EARG, "%sp2g_split must be 1, 2 or 3: %lld", "", c.p2g_split);
     g_launch = c;
     return PMPM_OK;
 }

@@ -125,7 +125,7 @@ def test_launch_config_round_trip_and_validation():
     assert autotune.get_config() == cfg
     for bad in ({"bucket_wgs": 1001}, {"bucket_wgs": 32768}, {"block_wgs": 4096}, {"p2g_pair": 3},
                 {"const_threads": 512}, {"p2g_waves": 3}, {"g2p_threads": 64}, {"cell_block": 6},
-                {"p2g_split": 3}):
+                {"p2g_split": 4}):
         with pytest.raises(RuntimeError):
             autotune.set_config(bad)
     # a rejected setting leaves the previous one in place

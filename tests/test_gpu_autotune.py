@@ -49,7 +49,8 @@ def test_single_particle_p2g_matches_paired_to_rounding():
             assert rel_err(getattr(b, k), getattr(a, k)) <= 1e-6, (cfg, k)
 
 
-LAYOUTS = [{"cell_block": 8}, {"p2g_split": 2}, {"p2g_split": 2, "p2g_pair": 1}]
+LAYOUTS = [{"cell_block": 8}, {"p2g_split": 2}, {"p2g_split": 2, "p2g_pair": 1}, {"p2g_split": 3},
+           {"p2g_split": 3, "p2g_pair": 1}]
 
 
 def uneven_spec(dtype, seed=5):
