@@ -50,6 +50,18 @@ namespace {
 constexpr int BLK = 4;                 // cells per bucket edge (= nodes per grid-block edge)
 constexpr int AR = BLK + 3;            // arena edge: nodes [4b-1, 4b+5]
 constexpr int AR3 = AR * AR * AR;      // 343
+// Padded LDS arenas of the lane-per-cell scatters: node (x, y, z) at entry
+// x PAD_XS + 7 y + z, and lane l owning cell (x, y, z) = ((l >> 2) & 3,
+// l >> 4, l & 3) of its bucket (pad_cell).  The 8 lanes of one 128-byte LDS
+// cycle of a 16-byte access (fixed y, x in {2k, 2k+1}, z 0..3) then reach 8
+// distinct entries mod 8, and the 16 lanes of an 8-byte access 16 distinct
+// entries mod 16, so a frame offset's update of the lanes' own cells runs
+// without bank conflicts (the dense stride of 49, in cell order, conflicts
+// two ways).
+constexpr int PAD_XS = AR * AR + 3;
+constexpr int PAD3 = AR * PAD_XS;        // entries of one padded arena
+__host__ __device__ constexpr int pad_cell(int l) { return (((l >> 2) & 3) << 4) | ((l >> 4) << 2) | (l & 3); }
+__host__ __device__ constexpr int pad_entry(int node) { return (node / (AR * AR)) * PAD_XS + node % (AR * AR); }
 constexpr int ARZ8 = 2 * BLK + 3;      // z edge of an 8-cell-long block arena: nodes [8c-1, 8c+9]
 constexpr int AR3B = AR * AR * ARZ8;   // 539
 constexpr int WG = 256;                // threads per work-group
@@ -1421,8 +1433,8 @@ k_p2g_cb8(StepArgs<T> A) {
 // With one arena per wavefront a bucket costs its fullest cell's passes
 // however empty the other lanes are, and a torn or compressed body leaves
 // cells unevenly filled (1 to 16 particles).  Here a cell of p >= 2 passes
-// is split: lane l runs the first a = ceil(p / 2) passes of its own cell l
-// into arena 0 (the lanes start in k_p2g's regular cell-per-lane pattern),
+// is split: a lane runs the first a = ceil(p / 2) passes of its own cell
+// into arena 0 (one lane per cell, as in k_p2g),
 // and the other p - a passes go, into arena 1, to a lane whose own load is
 // small: the lanes ranked by a ascending take the cells ranked by p - a
 // descending, rank for rank (ties by index, so the order is fixed).  A lane
@@ -1435,22 +1447,18 @@ k_p2g_cb8(StepArgs<T> A) {
 // channels' share of the arithmetic, and the smaller per-wavefront state
 // lets four wavefronts share a SIMD (128 VGPRs; k_p2g holds 168, three).
 // The arenas are summed in a fixed order and stored to the bucket's slot as
-// k_p2g stores it.
-// The arenas' x stride is padded from 49 to CHAN_XS = 52 entries and lane l
-// owns cell (x, y, z) = ((l >> 2) & 3, l >> 4, l & 3): the 8 lanes of one
-// 128-byte LDS cycle of a 16-byte access (fixed y, x in {2k, 2k+1}, z 0..3)
-// then reach 8 distinct entries mod 8, and the 16 lanes of an 8-byte
-// access 16 distinct entries mod 16, so a frame offset's update of the
-// lanes' own cells runs without bank conflicts (the dense stride, in cell
-// order, conflicts two ways).
-constexpr int CHAN_XS = AR * AR + 3;
-constexpr int CHAN_E = 2 * AR * CHAN_XS;  // entries of an arena pair
+// k_p2g stores it.  BAL splits the channels as (mv_x, mv_y) and (m, mv_z,
+// sum w d, sum w) instead, which evens out the two wavefronts' arithmetic
+// (14 FMAs per frame node and particle pair against 4 in the quadratic /
+// cubic split; 8 against 10 balanced).  The arenas are padded (PAD_XS) and
+// lane l owns cell pad_cell(l), free of bank conflicts among own cells.
+constexpr int CHAN_E = 2 * AR * PAD_XS;   // entries of an arena pair
 
 template <typename T, int NP, bool BAL>
 __global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
 k_p2g_chan(StepArgs<T> A) {
-    // arena pairs stacked along x: arena w's node (lx, ly, lz) is entry
-    // (w * 7 + lx) * CHAN_XS + ly * 7 + lz
+    // padded arena pairs stacked along x: arena w's node (lx, ly, lz) is
+    // entry (w * 7 + lx) * PAD_XS + ly * 7 + lz
     __shared__ __align__(16) T s_q[CHAN_E][4];
     __shared__ __align__(16) T s_c[CHAN_E][2];
     __shared__ int s_item[2][128];          // per wavefront: second items by rank (first record, records)
@@ -1467,7 +1475,7 @@ k_p2g_chan(StepArgs<T> A) {
         const int bk = A.alist[it.i];
         const int loc = (int)(bk % A.NBs);
         const int bx = loc / (nb * nb), by = (loc / nb) % nb, bz = loc % nb;
-        const long long cell = ((long long)bk << 6) + ((((lane >> 2) & 3) << 4) | ((lane >> 4) << 2) | (lane & 3));
+        const long long cell = ((long long)bk << 6) + pad_cell(lane);
         const int c0 = A.start[cell], n = A.start[cell + 1] - c0;
         const int pp = (n + NP - 1) / NP;
         const int pa = pp >= 2 ? (pp + 1) >> 1 : pp, pb = pp - pa;
@@ -1508,11 +1516,11 @@ k_p2g_chan(StepArgs<T> A) {
             }
             if (r < nx) {
                 if constexpr (BAL) {
-                    if (wv == 0) p2g_pass<T, NP, AR, CH_XY, CHAN_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
-                    else p2g_pass<T, NP, AR, CH_MZC, CHAN_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
+                    if (wv == 0) p2g_pass<T, NP, AR, CH_XY, PAD_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                    else p2g_pass<T, NP, AR, CH_MZC, PAD_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
                 } else {
-                    if (wv == 0) p2g_pass<T, NP, AR, CH_QUAD, CHAN_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
-                    else p2g_pass<T, NP, AR, CH_CUBIC, CHAN_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
+                    if (wv == 0) p2g_pass<T, NP, AR, CH_QUAD, PAD_XS>(A, ca + r, nx - r, org, s_q, nullptr, wa);
+                    else p2g_pass<T, NP, AR, CH_CUBIC, PAD_XS>(A, ca + r, nx - r, org, nullptr, s_c, wa);
                 }
                 r += NP;
             }
@@ -1522,7 +1530,7 @@ k_p2g_chan(StepArgs<T> A) {
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < 6 * AR3; l += 128) {
             const int node = l / 6, c = l - node * 6;
-            const int lx = node / (AR * AR), e0 = lx * CHAN_XS + (node - lx * AR * AR), e1 = e0 + AR * CHAN_XS;
+            const int e0 = pad_entry(node), e1 = e0 + PAD3;
             if constexpr (BAL) {
                 // (m, mv_z, sum w d, sum w) in s_q, (mv_x, mv_y) in s_c
                 const int e = c == 0 ? 0 : (c >= 3 ? c - 2 : c - 1);
@@ -2105,11 +2113,11 @@ template <typename T> __device__ __forceinline__ void pcg_mute(PcgPart<T>& q) {
 // are summed in registers, then one read-modify-write per node, in program
 // order, software-pipelined as the P2G scatter (the next node's entry is
 // read right after this node's write).
-template <typename T, int NP, bool CLAMP = false>
+template <typename T, int NP, bool CLAMP = false, int XS = AR * AR>
 __device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], int l0, const int (*lp)[4], const PcgPart<T> (&q)[NP]) {
     auto at = [&](int ox, int oy, int oz) {
-        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
-        else return l0 + (ox * AR + oy) * AR + oz;
+        if constexpr (CLAMP) return lp[0][ox] * XS + lp[1][oy] * AR + lp[2][oz];
+        else return l0 + ox * XS + oy * AR + oz;
     };
     T c0, c1, c2, c3;
     {
@@ -2161,11 +2169,11 @@ __device__ __forceinline__ void pcg_drive_scatter(T (*sq)[4], int l0, const int 
 // arena, software-pipelined as above.  Both halves are separable: per
 // (ox, oy) the gather keeps sum_z N_z p and sum_z N'_z p, and the scatter
 // forms N_z (N'_x N_y g_x + N_x N'_y g_y) + N'_z (N_x N_y g_z).
-template <typename T, int NP, bool CLAMP = false>
+template <typename T, int NP, bool CLAMP = false, int XS = AR * AR>
 __device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, int l0, const int (*lp)[4], const PcgPart<T> (&q)[NP]) {
     auto at = [&](int ox, int oy, int oz) {
-        if constexpr (CLAMP) return (lp[0][ox] * AR + lp[1][oy]) * AR + lp[2][oz];
-        else return l0 + (ox * AR + oy) * AR + oz;
+        if constexpr (CLAMP) return lp[0][ox] * XS + lp[1][oy] * AR + lp[2][oz];
+        else return l0 + ox * XS + oy * AR + oz;
     };
     T g[NP][3];
 #pragma unroll
@@ -2224,7 +2232,7 @@ __device__ __forceinline__ void pcg_apply_scatter(const T* sp, T* sa, int l0, co
 // frame).  A particle beyond the grid (its frame clamped onto the arena's
 // edge, which a second particle of the cell need not share) is scattered
 // alone afterwards, with its frame's node positions clamped one by one.
-template <typename T, bool APPLY>
+template <typename T, bool APPLY, int XS = AR * AR>
 __device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, bool two, const int org[3], const T* sp, T* arena) {
     PcgPart<T> q[2];
     pcg_load<T, !APPLY>(A, i0, A.ng, q[0]);
@@ -2234,9 +2242,9 @@ __device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, boo
     const bool alone[2] = {clamped, two && (!same || clamped)};
     if (clamped) pcg_mute(q[0]);
     if (!same || clamped) pcg_mute(q[1]);
-    const int l0 = frame_origin<AR>(q[0].bcb, org, 0);
-    if (APPLY) pcg_apply_scatter<T, 2>(sp, arena, l0, nullptr, q);
-    else pcg_drive_scatter<T, 2>((T(*)[4])arena, l0, nullptr, q);
+    const int l0 = frame_origin<AR, XS>(q[0].bcb, org, 0);
+    if (APPLY) pcg_apply_scatter<T, 2, false, XS>(sp, arena, l0, nullptr, q);
+    else pcg_drive_scatter<T, 2, false, XS>((T(*)[4])arena, l0, nullptr, q);
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
         if (!alone[j]) continue;
@@ -2244,8 +2252,8 @@ __device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, boo
         pcg_load<T, !APPLY>(A, i0 + j, A.ng, q1[0]);
         int lp[3][4];
         frame_nodes<AR>(q1[0].bcb, org, 0, lp);
-        if (APPLY) pcg_apply_scatter<T, 1, true>(sp, arena, 0, lp, q1);
-        else pcg_drive_scatter<T, 1, true>((T(*)[4])arena, 0, lp, q1);
+        if (APPLY) pcg_apply_scatter<T, 1, true, XS>(sp, arena, 0, lp, q1);
+        else pcg_drive_scatter<T, 1, true, XS>((T(*)[4])arena, 0, lp, q1);
     }
 }
 
@@ -2256,7 +2264,7 @@ __device__ __forceinline__ void pcg_pair(const StepArgs<T>& A, long long i0, boo
 template <typename T>
 __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
     constexpr int W = sizeof(T) == 4 ? 4 : 2;
-    __shared__ __align__(16) T s_q[W][AR3][4];
+    __shared__ __align__(16) T s_q[W][PAD3][4];   // padded arenas, lane l on cell pad_cell(l)
     const int nact = A.misc[0];
     const int nb = A.nb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -2264,23 +2272,23 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_drive(StepArgs<T> A) {
         const int bk = A.alist[it.i];
         const int bx = bk / (nb * nb), by = (bk / nb) % nb, bz = bk % nb;
         const int org[3] = {BLK * bx - 1, BLK * by - 1, BLK * bz - 1};
-        for (int l = threadIdx.x; l < W * 4 * AR3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < W * 4 * PAD3; l += P2G_WG) (&s_q[0][0][0])[l] = T(0);
         __syncthreads();
-        const long long cell = ((long long)bk << 6) + lane;
+        const long long cell = ((long long)bk << 6) + pad_cell(lane);
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
         for (int r = 2 * wv; r < rounds; r += 2 * W) {
             if (wv >= W || r >= cnt) continue;
-            pcg_pair<T, false>(A, c0 + r, r + 1 < cnt, org, nullptr, &s_q[wv][0][0]);
+            pcg_pair<T, false, PAD_XS>(A, c0 + r, r + 1 < cnt, org, nullptr, &s_q[wv][0][0]);
         }
         __syncthreads();
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < 4 * AR3; l += P2G_WG) {
-            const int node = l / 4, c = l - node * 4;
-            T v = s_q[0][node][c];
+            const int node = l / 4, c = l - node * 4, e = pad_entry(node);
+            T v = s_q[0][e][c];
 #pragma unroll
-            for (int w = 1; w < W; ++w) v += s_q[w][node][c];
+            for (int w = 1; w < W; ++w) v += s_q[w][e][c];
             dst[6 * node + c] = v;
         }
         __syncthreads();
@@ -2295,8 +2303,8 @@ template <typename T, bool DIRECT>
 __global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
     if (A.pscal[PCG_DONE] != 0.0) return;
     constexpr int W = sizeof(T) == 4 ? 4 : 2;
-    __shared__ T s_p[AR3];
-    __shared__ T s_a[W][AR3];
+    __shared__ T s_p[PAD3];                // padded, lane l on cell pad_cell(l)
+    __shared__ T s_a[W][PAD3];
     const int nact = A.misc[0];
     const int nb = A.nb, ng = A.ng;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -2311,24 +2319,25 @@ __global__ void __launch_bounds__(P2G_WG) k_pcg_apply(StepArgs<T> A) {
             // reference's clamped node index)
             const long long node = node_lin(org[0] + lx, org[1] + ly, org[2] + lz, ng);
             const T* v = A.gv + 4 * node;
-            s_p[l] = DIRECT ? v[2] : v[1] / A.pdiag[2 * node] + beta * v[2];
+            s_p[pad_entry(l)] = DIRECT ? v[2] : v[1] / A.pdiag[2 * node] + beta * v[2];
         }
-        for (int l = threadIdx.x; l < W * AR3; l += P2G_WG) (&s_a[0][0])[l] = T(0);
+        for (int l = threadIdx.x; l < W * PAD3; l += P2G_WG) (&s_a[0][0])[l] = T(0);
         __syncthreads();
-        const long long cell = ((long long)bk << 6) + lane;
+        const long long cell = ((long long)bk << 6) + pad_cell(lane);
         const int c0 = A.start[cell], cnt = A.start[cell + 1] - c0;
         int rounds = cnt;
         for (int o = 32; o > 0; o >>= 1) rounds = max(rounds, __shfl_xor(rounds, o));
         for (int r = 2 * wv; r < rounds; r += 2 * W) {
             if (wv >= W || r >= cnt) continue;
-            pcg_pair<T, true>(A, c0 + r, r + 1 < cnt, org, s_p, s_a[wv]);
+            pcg_pair<T, true, PAD_XS>(A, c0 + r, r + 1 < cnt, org, s_p, s_a[wv]);
         }
         __syncthreads();
         T* dst = A.arena + (size_t)it.i * (6 * AR3);
         for (int l = threadIdx.x; l < AR3; l += P2G_WG) {
-            T v = s_a[0][l];
+            const int e = pad_entry(l);
+            T v = s_a[0][e];
 #pragma unroll
-            for (int w = 1; w < W; ++w) v += s_a[w][l];
+            for (int w = 1; w < W; ++w) v += s_a[w][e];
             dst[6 * l] = v;
         }
         __syncthreads();
