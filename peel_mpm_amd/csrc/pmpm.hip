@@ -1978,14 +1978,43 @@ __global__ void __launch_bounds__(NT) k_g2p(StepArgs<T> A) {
             if (live && (lane == 0 || kbprev != kb)) A.cbflag[kb] = 1;
         };
         // (the lanes of a wavefront run these loops together, live or not,
-        // so carry's wavefront operations see every lane)
+        // so carry's wavefront operations see every lane).  The prefetched
+        // particles are all computed before their counts are taken, and
+        // their count atomics are all issued before any result is used, so
+        // one atomic round trip covers them.
+        int nkk[G2P_PREFETCH];
+        bool offk[G2P_PREFETCH], livek[G2P_PREFETCH];
 #pragma unroll
         for (int k = 0; k < G2P_PREFETCH; ++k) {
-            const bool live = (int)threadIdx.x + k * NT < cnt;
-            int nk = -1;
-            bool off = false;
-            if (live) particle(pf[k], xf[k], Ff[k], nk, off);
-            if (A.bin_next) carry(pf[k], nk, live, off);
+            livek[k] = (int)threadIdx.x + k * NT < cnt;
+            nkk[k] = -1;
+            offk[k] = false;
+            if (livek[k]) particle(pf[k], xf[k], Ff[k], nkk[k], offk[k]);
+        }
+        if (A.bin_next) {
+            const int lane = threadIdx.x & 63;
+            bool anyoff = false;
+            int base[G2P_PREFETCH], first[G2P_PREFETCH];
+#pragma unroll
+            for (int k = 0; k < G2P_PREFETCH; ++k) {
+                anyoff = anyoff || offk[k];
+                bool head;
+                int len;
+                wave_run(nkk[k], livek[k], lane, head, first[k], len);
+                base[k] = 0;
+                if (head) base[k] = atomicAdd(&A.ccount[nkk[k]], len);
+            }
+            if (__ballot(anyoff) != 0 && lane == 0) A.cmisc[5] = 1;
+#pragma unroll
+            for (int k = 0; k < G2P_PREFETCH; ++k) {
+                const int b = __shfl(base[k], first[k]);
+                if (livek[k]) {
+                    A.ckey[pf[k]] = nkk[k];
+                    A.cslot[pf[k]] = b + (lane - first[k]);
+                }
+                const int kb = nkk[k] >> 6, kbprev = __shfl_up(kb, 1);
+                if (livek[k] && (lane == 0 || kbprev != kb)) A.cbflag[kb] = 1;
+            }
         }
         for (int i0 = (int)(threadIdx.x & ~63u) + G2P_PREFETCH * NT; i0 < cnt; i0 += NT) {
             const int i = i0 + (int)(threadIdx.x & 63);
