@@ -79,9 +79,11 @@ PCG_GROUP, PCG_ITER_GROUP = 4, 5    # pmpm_timing groups: the implicit damage so
 # wavefronts split by channel (p2g_split 2, p2g_waves then unused) 0.2607
 # against 0.2625 ms, and the balanced channel split (p2g_split 3: mv_x, mv_y
 # on one wavefront, m, mv_z and the cubic pair on the other) 0.2528 against
-# 0.2559 ms.
-PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 2,
-                 "g2p_threads": 256, "cell_block": 4, "p2g_split": 3}
+# 0.2559 ms; the constitutive kernel on a second stream beside the binning
+# (const_stream 2, with 256-thread work-groups) 0.2462-0.2467 against
+# 0.2527 ms on the launch stream after it.
+PINNED_LAUNCH = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2,
+                 "g2p_threads": 256, "cell_block": 4, "p2g_split": 3, "const_stream": 2}
 
 FRAME = 10                          # substeps per run_frame call in the timed legs
 
@@ -182,12 +184,16 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
         cs.append(c)
     torch.cuda.synchronize()
     n_q, n_c = round(sum(qs) / len(qs)), round(sum(cs) / len(cs))
-    ms = (C.c_double * 6)()
-    cnt = (C.c_int64 * 6)()
-    _lib.check(L.pmpm_timing_read(ms, cnt, 6), "pmpm_timing_read")
+    ms = (C.c_double * 7)()
+    cnt = (C.c_int64 * 7)()
+    _lib.check(L.pmpm_timing_read(ms, cnt, 7), "pmpm_timing_read")
     L.pmpm_timing(0)
     sim.use_graph = use_graph
     groups = {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS + ("pcg_solve", "pcg_iteration"))}
+    # with const_stream 2 the constitutive kernel is timed apart, on the
+    # second stream (group 6), beside the binning: stress+P2G per step is
+    # still its duration plus P2G's
+    groups["stress_p2g"] = (ms[1] + ms[6], cnt[1])
     if count_cg:
         restart()
         ran = 0

@@ -277,6 +277,16 @@ typedef struct {
                                p2g_waves is then not used) or 3 (as 2, with the channels
                                balanced between the wavefronts: mv_x, mv_y on one and m, mv_z
                                and the cubic pair on the other); default 1 */
+    int32_t const_stream;   /* where the explicit-damage step runs its constitutive kernel:
+                               1 (on the launch stream after the binning; its P2G records at
+                               the particles' sorted positions) or 2 (on a second stream of
+                               the library's, one per device, beside the binning's scan and
+                               sort; the records in particle order, P2G reading them through
+                               the sort permutation).  The PCG damage mode, steps that keep
+                               their records for the adjoint, pmpm_substep_noalloc and a
+                               batched step that counts its cells afresh with a per-scene
+                               damage gate take 1.  Steps enqueued from several host threads
+                               on one device share that device's second stream.  Default 1 */
 } pmpm_launch_config;
 /* NULL restores the defaults.  Out-of-range values: PMPM_EARG, nothing changed. */
 int pmpm_set_launch_config(const pmpm_launch_config *cfg);
@@ -339,9 +349,11 @@ int pmpm_sym_eig(const void *A, int32_t dtype, int64_t n, void *evals, void *V, 
 /* Live kernel timing with HIP events on the launch stream (for benchmarks):
  * pmpm_timing(1) clears and starts recording around each kernel group
  * (0 binning, 1 stress+P2G, 2 grid update, 3 G2P, 4 the PCG damage solve,
- * 5 one CG iteration within it); pmpm_timing_read synchronises and returns
- * each group's summed milliseconds and launch count (n <= 6 groups).  Adds
- * two event records per group per step. */
+ * 5 one CG iteration within it, 6 the constitutive kernel when it runs on
+ * the second stream, launch shape const_stream 2: group 1 then holds P2G
+ * alone); pmpm_timing_read synchronises and returns each group's summed
+ * milliseconds and launch count (n <= 7 groups).  Adds two event records
+ * per group per step. */
 int pmpm_timing(int32_t enable);
 int pmpm_timing_read(double *total_ms, int64_t *launches, int32_t n);
 

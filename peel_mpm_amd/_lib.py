@@ -68,7 +68,7 @@ _OUT_PTRS = ("x", "v", "F", "C", "lap", "d", "D_hist", "phi", "phi_shear", "tau_
 class LaunchConfig(C.Structure):
     _fields_ = [("bucket_wgs", C.c_int32), ("block_wgs", C.c_int32), ("p2g_pair", C.c_int32),
                 ("const_threads", C.c_int32), ("p2g_waves", C.c_int32), ("g2p_threads", C.c_int32),
-                ("cell_block", C.c_int32), ("p2g_split", C.c_int32)]
+                ("cell_block", C.c_int32), ("p2g_split", C.c_int32), ("const_stream", C.c_int32)]
 
 
 class FrameIn(C.Structure):

@@ -26,7 +26,11 @@ the library (``pmpm_set_launch_config``, include/pmpm.h):
   (by channels: one the quadratic, one the cubic, each with two arenas,
   the cells of two or more passes split over them and packed onto the
   lanes by load) or 3 (as 2 with the channels balanced: mv_x and mv_y on
-  one wavefront, m, mv_z and the cubic pair on the other).
+  one wavefront, m, mv_z and the cubic pair on the other);
+* ``const_stream``: where the explicit-damage step runs its constitutive
+  kernel -- 1 (on the launch stream after the binning) or 2 (on a second
+  stream beside the binning's scan and sort, its P2G records in particle
+  order, read by P2G through the sort permutation).
 
 The bucket (4x4x4 cells, one cell per lane of a 64-wide wavefront) stays
 the unit of the binning and of the lane-per-cell scatter.
@@ -57,9 +61,10 @@ SPACE = {
     "block_wgs": (512, 1024, 2048),
     "cell_block": (4, 8),
     "p2g_split": (1, 2, 3),
+    "const_stream": (1, 2),
 }
 DEFAULT = {"bucket_wgs": 2048, "block_wgs": 2048, "p2g_pair": 2, "const_threads": 256, "p2g_waves": 2,
-           "g2p_threads": 256, "cell_block": 4, "p2g_split": 1}
+           "g2p_threads": 256, "cell_block": 4, "p2g_split": 1, "const_stream": 1}
 _CACHE: dict = {}
 
 

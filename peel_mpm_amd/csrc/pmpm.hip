@@ -94,7 +94,7 @@ inline int nblk(int ng) { return (ng + BLK - 1) / BLK; }
 // kernel group, summed on read.  Groups: 0 binning, 1 p2g, 2 grid, 3 g2p,
 // 4 the PCG damage solve (drive, assembly, iterations, interpolation),
 // 5 one CG iteration of it (operator, fold, reductions, update).
-constexpr int NTIMED = 6;
+constexpr int NTIMED = 7;
 struct Timing {
     bool on = false;
     std::vector<hipEvent_t> ev[NTIMED];   // start/stop pairs
@@ -133,7 +133,7 @@ struct TimedScope {
 inline size_t al(size_t v) { return (v + 255) & ~size_t(255); }
 
 // Process-wide launch shape (pmpm_set_launch_config); defaults below.
-pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1};
+pmpm_launch_config g_launch = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1, 1};
 
 // Workspace layout (byte offsets).  The step keeps count, bflag, mark and
 // misc at zero between steps: whoever reads them clears them.  count/start
@@ -659,6 +659,7 @@ template <typename T> struct StepArgs {
     uint8_t* active;
     const int *start, *alist, *slotof, *blist, *perm, *misc, *key, *inv;
     T *rec, *dsrt;              // P2G records and post-update d at sorted positions
+    int rec_pp;                 // 1: rec and dsrt in particle order instead (P2G reads them through perm)
     // the carried binning: with bin_next, G2P takes the next step's cell keys
     // and slots (ckey, cslot) and counts into the carried set (ccount, cbflag,
     // cmisc[5]) from the positions it writes
@@ -991,7 +992,7 @@ __global__ void __launch_bounds__(WG) k_const(StepArgs<T> A) {
     const T xp[3] = {A.x[3 * p], A.x[3 * p + 1], A.x[3 * p + 2]};
     const T mass = fld<PMPM_U_MASS>(A, A.mass, p), vol0 = fld<PMPM_U_VOL0>(A, A.vol0, p);
     const int bc = A.bc_id ? A.bc_id[p] : 0;
-    const long long i = A.inv[p];
+    const long long i = A.rec_pp ? p : (long long)A.inv[p];
     // a batched step gates damage per scene; the scene is in the cell key
     const bool dmg_on = !DAMAGE || A.scene_dmg == nullptr || A.scene_dmg[(A.key[p] >> 6) / A.NBs] != 0;
     T d = q.d, J;
@@ -1044,6 +1045,8 @@ template <typename T> struct P2GPart {
 
 template <typename T, int CH = CH_ALL>
 __device__ __forceinline__ void p2g_load(const StepArgs<T>& A, long long i, int ng, P2GPart<T>& q, int sh[3]) {
+    // sorted position i; records in particle order are read through perm
+    if (A.rec_pp) i = A.perm[i];
     const T* r = A.rec + (size_t)P2G_REC * i;
     const T Xp[3] = {scaled(r[0], A.inv_dx), scaled(r[1], A.inv_dx), scaled(r[2], A.inv_dx)};
     if constexpr (CH != CH_CUBIC) {
@@ -2717,19 +2720,57 @@ __global__ void __launch_bounds__(WG) k_stencil(const T* __restrict__ x, long lo
 // ------------------------------------------------------------------ host side
 inline unsigned nblocks(long long n) { return (unsigned)((n + WG - 1) / WG); }
 
+// The second stream of a device (launch shape const_stream 2): the
+// constitutive kernel runs on it beside the binning.  fork is recorded on the
+// launch stream and waited for by the second stream; join the other way.
+// Default priority, non-blocking (no implicit ordering with the null stream:
+// the two events order it).  Made once per device and kept until the process
+// ends.  g_side_mu also serialises a step's fork ... join enqueueing, so the
+// two events are never re-recorded by another host thread in between.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int SIDE_MAX_DEVICES = 64;
+std::mutex g_side_mu;
+SideStream g_side[SIDE_MAX_DEVICES];
+
+int side_stream_locked(SideStream** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SIDE_MAX_DEVICES) return fail(PMPM_EARG, "%sdevice ordinal out of range: %lld", "", dev);
+    SideStream& ss = g_side[dev];
+    if (!ss.s) {
+        HIPCHK(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
+        HIPCHK(hipStreamCreateWithFlags(&ss.s, hipStreamNonBlocking));
+    }
+    *out = &ss;
+    return PMPM_OK;
+}
+
+int side_stream(SideStream** out) {
+    std::lock_guard<std::mutex> lk(g_side_mu);
+    return side_stream_locked(out);
+}
+
 // The grid pass over the P2G arenas of the launch's cell block (A.cb).
 template <typename T> void launch_grid(const StepArgs<T>& A, int wgs, hipStream_t st) {
     if (A.cb == 8) hipLaunchKernelGGL((k_grid<T, 2>), dim3(wgs), dim3(WG), 0, st, A);
     else hipLaunchKernelGGL((k_grid<T, 1>), dim3(wgs), dim3(WG), 0, st, A);
 }
 
-// Constitutive update (particle order) then P2G (bucket order).
-template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
+// Constitutive update (particle order).
+template <typename T> void launch_const(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
     const int ct = g_launch.const_threads;
     const dim3 PB((unsigned)((A.n + ct - 1) / ct)), B(ct);
     if (damage && exact_sigma) hipLaunchKernelGGL((k_const<T, true, true>), PB, B, 0, st, A);
     else if (damage) hipLaunchKernelGGL((k_const<T, true, false>), PB, B, 0, st, A);
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
+}
+
+// P2G (bucket order) from the constitutive kernel's records.
+template <typename T> void launch_p2g(const StepArgs<T>& A, hipStream_t st) {
     const dim3 G(g_launch.bucket_wgs);
     if (A.cb == 8) {
         if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g_cb8<T, 1>), G, dim3(64), 0, st, A);
@@ -2763,6 +2804,12 @@ template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bo
         if (g_launch.p2g_pair == 1) hipLaunchKernelGGL((k_p2g<T, 1, 2>), G, dim3(128), 0, st, A);
         else hipLaunchKernelGGL((k_p2g<T, 2, 2>), G, dim3(128), 0, st, A);
     }
+}
+
+// Constitutive update (particle order) then P2G (bucket order).
+template <typename T> void launch_transfer(const StepArgs<T>& A, bool damage, bool exact_sigma, hipStream_t st) {
+    launch_const<T>(A, damage, exact_sigma, st);
+    launch_p2g<T>(A, st);
 }
 
 // The binning.  pre: the carried count of the previous step's G2P (count2,
@@ -2918,19 +2965,50 @@ int check_step(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
 
 // carry: the workspace may carry binning between steps (not the
 // no-allocation scratch, which is cleared every call).
+// side_ok: the step may put its constitutive kernel on the device's second
+// stream (launch shape const_stream 2); not for the no-allocation scratch.
 template <typename T>
 int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmpm_particles_out* out, char* ws, hipStream_t st,
-               int exact_sigma, bool host_poll = true, bool carry = true) {
+               int exact_sigma, bool host_poll = true, bool carry = true, bool side_ok = true) {
     Layout L(in->n, p->n_grid, p->dtype, p->n_scenes);
     const bool pre = carry && carry_take(ws, p->prebinned != 0, in->x, in->n, st);
-    int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
-                        p->n_scenes, L, ws, st, pre);
-    if (rc) return rc;
     StepArgs<T> A = make_args<T>(p, in, out, L, ws);
+    const bool pcg = p->do_damage && p->damage_solver == PMPM_DAMAGE_PCG && in->n > 0;
+    // The constitutive kernel beside the binning: it reads the particle
+    // arrays and writes its outputs and its records in particle order, none
+    // of which the binning touches; it reads the cell keys only for a
+    // batched damage gate, and only a fresh count (k_bin_count, in run_bin)
+    // writes them.  P2G, after the join, reads the records through perm.
+    const bool side = side_ok && g_launch.const_stream == 2 && in->n > 0 && !pcg && !p->keep_for_adjoint &&
+                      (pre || A.scene_dmg == nullptr);
+    if (side) {
+        std::lock_guard<std::mutex> lk(g_side_mu);
+        SideStream* ss = nullptr;
+        if (int rc = side_stream_locked(&ss)) return rc;
+        A.rec_pp = 1;
+        HIPCHK(hipEventRecord(ss->fork, st));
+        HIPCHK(hipStreamWaitEvent(ss->s, ss->fork, 0));
+        {
+            TimedScope ts(6, ss->s);
+            launch_const<T>(A, p->do_damage != 0, exact_sigma != 0, ss->s);
+        }
+        HIPCHK(hipEventRecord(ss->join, ss->s));
+        int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
+                            p->n_scenes, L, ws, st, pre);
+        if (rc) {
+            // the launch stream still joins the second stream before returning
+            (void)hipStreamWaitEvent(st, ss->join, 0);
+            return rc;
+        }
+        HIPCHK(hipStreamWaitEvent(st, ss->join, 0));
+    } else {
+        int rc = run_bin<T>((const T*)in->x, in->n, p->n_grid, p->inv_dx, p->stable_sort, (const long long*)p->scene_offsets,
+                            p->n_scenes, L, ws, st, pre);
+        if (rc) return rc;
+    }
     if (p->export_active)
         hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks(((long long)L.G + 15) / 16 < 1 ? 1 : ((long long)L.G + 15) / 16)),
                            dim3(WG), 0, st, (unsigned char*)(ws + L.active), (long long)L.G);
-    const bool pcg = p->do_damage && p->damage_solver == PMPM_DAMAGE_PCG && in->n > 0;
     if (pcg) {
         TimedScope ts(4, st);
         const dim3 PG(g_launch.bucket_wgs), PB(P2G_WG), GG(g_launch.block_wgs), GB(WG);
@@ -3008,6 +3086,8 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
             StepArgs<T> A2 = A;
             A2.d = A.od;
             launch_transfer<T>(A2, false, false, st);
+        } else if (side) {
+            launch_p2g<T>(A, st);      // the constitutive kernel ran beside the binning
         } else {
             launch_transfer<T>(A, p->do_damage != 0, exact_sigma != 0, st);
         }
@@ -3135,8 +3215,9 @@ int pmpm_last_step_stats(const void* ws, int64_t max_n, int32_t n_grid, int32_t 
 }
 
 int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
-    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1};
+    pmpm_launch_config c = {PERSIST_WG, GRID_WG_COUNT, 2, 256, 2, 256, 4, 1, 1};
     if (cfg) {
+        if (cfg->const_stream) c.const_stream = cfg->const_stream;
         if (cfg->cell_block) c.cell_block = cfg->cell_block;
         if (cfg->p2g_split) c.p2g_split = cfg->p2g_split;
         if (cfg->bucket_wgs) c.bucket_wgs = cfg->bucket_wgs;
@@ -3161,6 +3242,14 @@ int pmpm_set_launch_config(const pmpm_launch_config* cfg) {
         return fail(PMPM_EARG, "%scell_block must be 4 or 8: %lld", "", c.cell_block);
     if (c.p2g_split < 1 || c.p2g_split > 3)
         return fail(PMPM_EARG, "%sp2g_split must be 1, 2 or 3: %lld", "", c.p2g_split);
+    if (c.const_stream != 1 && c.const_stream != 2)
+        return fail(PMPM_EARG, "%sconst_stream must be 1 or 2: %lld", "", c.const_stream);
+    if (c.const_stream == 2) {
+        // the second stream of the current device, made here, outside any
+        // stream capture a later step may run under
+        SideStream* ss = nullptr;
+        if (int rc = side_stream(&ss)) return rc;
+    }
     g_launch = c;
     return PMPM_OK;
 }
@@ -3275,8 +3364,8 @@ int pmpm_substep_noalloc(const pmpm_step_params* p, const pmpm_particles_in* in,
     hipLaunchKernelGGL(k_zero_bytes, dim3(nblocks((nz / 16) < 1 ? 1 : nz / 16)), dim3(WG), 0, st, (unsigned char*)ws, nz);
     HIPCHK(hipGetLastError());
     t_untimed = true;
-    int rc = p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false)
-                                  : step_begin<float>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false);
+    int rc = p->dtype == PMPM_F64 ? step_begin<double>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false, false)
+                                  : step_begin<float>(p, in, out, (char*)ws, st, p->exact_sigma_eig, false, false, false);
     if (!rc)
         rc = p->dtype == PMPM_F64 ? step_end<double>(p, in, out, (char*)ws, st, false)
                                   : step_end<float>(p, in, out, (char*)ws, st, false);

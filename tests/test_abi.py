@@ -120,12 +120,12 @@ def test_launch_config_round_trip_and_validation():
     autotune.set_config(None)
     assert autotune.get_config() == autotune.DEFAULT
     cfg = {"bucket_wgs": 1024, "block_wgs": 512, "p2g_pair": 1, "const_threads": 128, "p2g_waves": 4, "g2p_threads": 512,
-           "cell_block": 8, "p2g_split": 2}
+           "cell_block": 8, "p2g_split": 2, "const_stream": 1}
     autotune.set_config(cfg)
     assert autotune.get_config() == cfg
     for bad in ({"bucket_wgs": 1001}, {"bucket_wgs": 32768}, {"block_wgs": 4096}, {"p2g_pair": 3},
                 {"const_threads": 512}, {"p2g_waves": 3}, {"g2p_threads": 64}, {"cell_block": 6},
-                {"p2g_split": 4}):
+                {"p2g_split": 4}, {"const_stream": 3}):
         with pytest.raises(RuntimeError):
             autotune.set_config(bad)
     # a rejected setting leaves the previous one in place

@@ -1,0 +1,110 @@
+"""The constitutive kernel on a second stream beside the binning (launch
+shape const_stream 2) against the same steps with it on the launch stream
+after the binning (const_stream 1).  Its P2G records move from the
+particles' sorted positions to particle order, and P2G reads them through
+the sort permutation in the same order, so every field agrees bit for bit:
+eager frames and frames replayed from a HIP graph (a moving collider, a
+timed pull, damage every second step), f_clamp, exact Cauchy eigenpairs,
+the config0 scene, a batched step replayed from a graph, and the PCG
+damage mode (which keeps the constitutive kernel on the launch stream)."""
+from __future__ import annotations
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+KEYS = ("x", "v", "F", "C", "d", "lap", "D_hist", "tau_last", "J_last", "phi", "phi_shear")
+# the balanced channel split and the other knobs at their config1 values
+SHAPE = {"bucket_wgs": 8192, "block_wgs": 1024, "p2g_pair": 2, "const_threads": 64, "p2g_waves": 2,
+         "g2p_threads": 256, "cell_block": 4, "p2g_split": 3}
+
+
+@pytest.fixture(autouse=True)
+def _restore_launch_shape():
+    from peel_mpm_amd import autotune
+    saved = autotune.get_config()
+    yield
+    autotune.set_config(saved)
+
+
+def _slab(dtype, **kw):
+    from peel_mpm_amd import AnisoMPM, VelocityBC, scenes, sphere_collider
+    spec = scenes.slab_block(n_grid=32, cells=(12, 4, 10), origin=(10, 8, 11), ppc=8, seed=4, dtype=dtype)
+    centre = lambda t: (0.50 + 30.0 * t, 0.43 - 20.0 * t, 0.5)
+    pull = lambda t: (0.0, 0.5 + 1500.0 * t, 0.05)
+    sim = scenes.make_solver(AnisoMPM, spec, device=DEV, **kw)
+    sim.particle_bc[0] = VelocityBC(sim.particle_bc[0].mask, pull)
+    sim.colliders.append(sphere_collider(centre, 0.07, mode="slip", friction=0.2))
+    return sim
+
+
+def _frames(stream, make, frames, shape=SHAPE):
+    from peel_mpm_amd import autotune
+    autotune.set_config(dict(shape, const_stream=stream))
+    sim = make()
+    t = 0.0
+    for n_sub in frames:
+        sim.run_frame(n_sub, t)
+        t += n_sub * sim.dt
+    torch.cuda.synchronize()
+    return sim
+
+
+def _same(a, b, label, keys=KEYS):
+    for k in keys:
+        assert torch.equal(getattr(a, k), getattr(b, k)), f"{label}: {k}"
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_second_stream_frames_match(dtype, use_graph):
+    print(f"const_stream frames {dtype} graph={use_graph}", flush=True)
+    make = lambda: _slab(dtype, damage_every=2, use_graph=use_graph)
+    a = _frames(1, make, (5, 4, 6))
+    b = _frames(2, make, (5, 4, 6))
+    _same(a, b, f"{dtype} graph={use_graph}")
+    assert float(b.D_hist.max()) > 0.0
+
+
+@pytest.mark.parametrize("shape", [SHAPE, {}], ids=["config1_shape", "default_shape"])
+def test_second_stream_with_fclamp_and_exact_eigenpairs(shape):
+    print(f"const_stream f_clamp exact_sigma {shape}", flush=True)
+    make = lambda: _slab(torch.float32, f_clamp=(0.9, 1.1), exact_sigma_eig=True)
+    a = _frames(1, make, (4, 4), shape)
+    b = _frames(2, make, (4, 4), shape)
+    _same(a, b, "f_clamp, exact eigenpairs")
+
+
+def test_second_stream_config0_scene():
+    from peel_mpm_amd import AnisoMPM, scenes
+    print("const_stream config0 scene", flush=True)
+    make = lambda: scenes.make_solver(AnisoMPM, scenes.config0(torch.float32), device=DEV)
+    a = _frames(1, make, (10, 10))
+    b = _frames(2, make, (10, 10))
+    _same(a, b, "config0")
+
+
+def test_second_stream_batched_graph():
+    from peel_mpm_amd import autotune
+    from tests.test_gpu_batched import build_batched, small_scenes
+    print("const_stream batched graph", flush=True)
+    out = []
+    for stream in (1, 2):
+        autotune.set_config(dict(SHAPE, const_stream=stream))
+        b = build_batched(small_scenes(), use_graph=True)
+        for s in range(6):
+            b.substep(s * b.dt)
+        torch.cuda.synchronize()
+        out.append(b)
+    for i in range(4):
+        for k in ("x", "v", "F", "C", "d", "lap", "D_hist"):
+            assert torch.equal(out[0].scene(i, k), out[1].scene(i, k)), (i, k)
+
+
+def test_pcg_damage_mode_keeps_the_launch_stream():
+    print("const_stream pcg", flush=True)
+    make = lambda: _slab(torch.float32, damage_solver="pcg")
+    a = _frames(1, make, (4,))
+    b = _frames(2, make, (4,))
+    _same(a, b, "pcg")
