@@ -192,7 +192,7 @@ def time_gpu(spec, steps, warmup, tune=True, prep=None, frame=FRAME, use_graph=F
     groups = {k: (ms[i], cnt[i]) for i, k in enumerate(KERNELS + ("pcg_solve", "pcg_iteration"))}
     # with const_stream 2 the constitutive kernel is timed apart, on the
     # second stream (group 6), beside the binning: stress+P2G per step is
-    # still its duration plus P2G's
+    # still its duration plus P2G's (the PCG mode keeps one stream)
     groups["stress_p2g"] = (ms[1] + ms[6], cnt[1])
     if count_cg:
         restart()

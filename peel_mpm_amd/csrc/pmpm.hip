@@ -2065,7 +2065,9 @@ template <typename T> __device__ __forceinline__ void cubic_slopes(const T Xp[3]
 }
 
 // Work-group sum of NCH doubles, stored as this group's partials
-// (channel-major, GRID_WG_COUNT per channel).  Fixed order: deterministic.
+// (channel-major, GRID_WG_COUNT per channel), in a fixed order within the
+// group; which node blocks a group holds follows the marked-block list,
+// whose order the atomic appends of its compaction set.
 template <int NCH> __device__ __forceinline__ void store_partials(double v[NCH], double* __restrict__ part) {
     __shared__ double s[NCH][WG / 64];
 #pragma unroll
@@ -2769,6 +2771,15 @@ template <typename T> void launch_const(const StepArgs<T>& A, bool damage, bool 
     else hipLaunchKernelGGL((k_const<T, false, false>), PB, B, 0, st, A);
 }
 
+// The PCG mode's constitutive update up to the damage drive and its solve
+// records (particle order).
+template <typename T> void launch_pcg_const(const StepArgs<T>& A, bool exact_sigma, hipStream_t st) {
+    const int ct = g_launch.const_threads;
+    const dim3 CG((unsigned)((A.n + ct - 1) / ct)), CB(ct);
+    if (exact_sigma) hipLaunchKernelGGL((k_pcg_const<T, true>), CG, CB, 0, st, A);
+    else hipLaunchKernelGGL((k_pcg_const<T, false>), CG, CB, 0, st, A);
+}
+
 // P2G (bucket order) from the constitutive kernel's records.
 template <typename T> void launch_p2g(const StepArgs<T>& A, hipStream_t st) {
     const dim3 G(g_launch.bucket_wgs);
@@ -3013,10 +3024,7 @@ int step_begin(const pmpm_step_params* p, const pmpm_particles_in* in, const pmp
         TimedScope ts(4, st);
         const dim3 PG(g_launch.bucket_wgs), PB(P2G_WG), GG(g_launch.block_wgs), GB(WG);
         const int nparts = g_launch.block_wgs;
-        const int ct = g_launch.const_threads;
-        const dim3 CG((unsigned)((A.n + ct - 1) / ct)), CB(ct);
-        if (exact_sigma) hipLaunchKernelGGL((k_pcg_const<T, true>), CG, CB, 0, st, A);
-        else hipLaunchKernelGGL((k_pcg_const<T, false>), CG, CB, 0, st, A);
+        launch_pcg_const<T>(A, exact_sigma != 0, st);
         hipLaunchKernelGGL(k_pcg_drive<T>, PG, PB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_fold<T, PCG_ASSEMBLE>), GG, GB, 0, st, A);
         hipLaunchKernelGGL((k_pcg_apply<T, true>), PG, PB, 0, st, A);            // A x0

@@ -5,8 +5,12 @@ particles' sorted positions to particle order, and P2G reads them through
 the sort permutation in the same order, so every field agrees bit for bit:
 eager frames and frames replayed from a HIP graph (a moving collider, a
 timed pull, damage every second step), f_clamp, exact Cauchy eigenpairs,
-the config0 scene, a batched step replayed from a graph, and the PCG
-damage mode (which keeps the constitutive kernel on the launch stream)."""
+the config0 scene and a batched step replayed from a graph.  The PCG
+damage mode keeps one stream under either setting; its solve's
+double-precision dot products sum the grid's node blocks in the
+order the binning's atomic appends list them, which varies from run to
+run, so its steps agree to the last bits of those scalars: 1e-12 in fp64,
+the 1e-5 parity bar in fp32."""
 from __future__ import annotations
 
 import pytest
@@ -56,6 +60,13 @@ def _same(a, b, label, keys=KEYS):
         assert torch.equal(getattr(a, k), getattr(b, k)), f"{label}: {k}"
 
 
+def _close(a, b, label, tol, keys=KEYS):
+    from tests.golden_cases import rel_err
+    for k in keys:
+        e = rel_err(getattr(a, k), getattr(b, k))
+        assert e <= tol, f"{label}: {k} {e:.3e}"
+
+
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 @pytest.mark.parametrize("use_graph", [False, True])
 def test_second_stream_frames_match(dtype, use_graph):
@@ -102,9 +113,28 @@ def test_second_stream_batched_graph():
             assert torch.equal(out[0].scene(i, k), out[1].scene(i, k)), (i, k)
 
 
-def test_pcg_damage_mode_keeps_the_launch_stream():
-    print("const_stream pcg", flush=True)
-    make = lambda: _slab(torch.float32, damage_solver="pcg")
-    a = _frames(1, make, (4,))
-    b = _frames(2, make, (4,))
-    _same(a, b, "pcg")
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_pcg_damage_mode_under_either_setting(dtype):
+    print(f"const_stream pcg {dtype}", flush=True)
+    make = lambda: _slab(dtype, damage_solver="pcg")
+    a = _frames(1, make, (4, 3))
+    b = _frames(2, make, (4, 3))
+    tol = 1e-12 if dtype == torch.float64 else 1e-5
+    _close(a, b, "pcg", tol)
+    (ia, ra), (ib, rb) = a.last_damage_solve(), b.last_damage_solve()
+    assert ia == ib and abs(ra - rb) <= 1e-6 * max(abs(ra), 1e-30)
+
+
+def test_pcg_stiff_solve_under_either_setting():
+    """A solve of several CG iterations per step (eta 1e-4, l0 = 2 dx)."""
+    print("const_stream stiff pcg", flush=True)
+
+    def make():
+        s = _slab(torch.float32, damage_solver="pcg")
+        s.eta.fill_(1e-4)
+        s.l0.fill_(2.0 * s.dx)
+        return s
+    a = _frames(1, make, (3,))
+    b = _frames(2, make, (3,))
+    _close(a, b, "stiff pcg", 1e-5)
+    assert a.last_damage_solve()[0] >= 2
