@@ -5,7 +5,8 @@ particles' sorted positions to particle order, and P2G reads them through
 the sort permutation in the same order, so every field agrees bit for bit:
 eager frames and frames replayed from a HIP graph (a moving collider, a
 timed pull, damage every second step), f_clamp, exact Cauchy eigenpairs,
-the config0 scene and a batched step replayed from a graph.  The PCG
+a collider callable on the host between the step's halves, the config0
+scene and a batched step, eager and replayed from a graph.  The PCG
 damage mode keeps one stream under either setting; its solve's
 double-precision dot products sum the grid's node blocks in the
 order the binning's atomic appends list them, which varies from run to
@@ -96,14 +97,29 @@ def test_second_stream_config0_scene():
     _same(a, b, "config0")
 
 
-def test_second_stream_batched_graph():
+def test_second_stream_host_collider_split_step():
+    """A collider callable runs on the host between the two halves of the
+    step (pmpm_substep_begin / _end): the fork and join sit in the first."""
+    print("const_stream host collider", flush=True)
+
+    def make():
+        s = _slab(torch.float32)
+        s.colliders.append(lambda pos, vs, t: torch.where(pos[:, 1:2] < 0.3, vs * 0.5, vs))
+        return s
+    a = _frames(1, make, (3, 3))
+    b = _frames(2, make, (3, 3))
+    _same(a, b, "host collider")
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_second_stream_batched(use_graph):
     from peel_mpm_amd import autotune
     from tests.test_gpu_batched import build_batched, small_scenes
-    print("const_stream batched graph", flush=True)
+    print(f"const_stream batched graph={use_graph}", flush=True)
     out = []
     for stream in (1, 2):
         autotune.set_config(dict(SHAPE, const_stream=stream))
-        b = build_batched(small_scenes(), use_graph=True)
+        b = build_batched(small_scenes(), use_graph=use_graph)
         for s in range(6):
             b.substep(s * b.dt)
         torch.cuda.synchronize()
